@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Headline benchmark: 3D Laplace FMM on one MI355X.
+
+Workload (the 4M-particle 3D configuration of the metric): N = 4,194,304
+particles, positions uniform in [0,1)^3 then charges uniform in [-1,1]
+(seeded numpy generator, positions drawn first), sources = targets, order
+p = 10 (11 terms), uniform Morton octree with leaves at level 6 (at most 64
+particles per leaf), FP64 throughout.  One step = one full solve through the
+C ABI with inputs and outputs resident in HBM: extent, keys, radix sort, tree
+levels, P2M, M2M, M2L+L2L, P2P+L2P and the scatter back to input order.
+
+Printed: one JSON line with particles/s, the accuracy against an FP64 direct
+sum on sampled targets (self term excluded), the M2L roofline (kernel time
+from HIP events recorded by the library on its stream during the timed
+steps) and a CPU baseline (the CPU oracle on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particles/sec (3D, 4M) at rel-L2 err ≤1e-6 vs FP64 direct; M2L MFMA util"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def m2l_flops_per_op(N):
+    """Algorithmic FP64 flops of one rotation-based M2L with N terms:
+    Rz and Ry forward and back (2 (6 Nx2 + sum_{n<N} (n+1)(4n+2))) plus the
+    axial translation (4 sum_{t=1..N} t^2)."""
+    nx2 = N * (N + 1) // 2
+    rot = sum((n + 1) * (4 * n + 2) for n in range(N))
+    return 2 * (6 * nx2 + rot) + 4 * sum(t * t for t in range(1, N + 1))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=4194304)
+    ap.add_argument("--terms", type=int, default=11)
+    ap.add_argument("--depth", type=int, default=7)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--sample", type=int, default=4096, help="targets checked against the direct sum")
+    ap.add_argument("--cpu-n", type=int, default=524288, help="particles in the CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build measures one GPU only (the path runs on a single MI355X); "
+              "--gpus %d / WORLD_SIZE=%d is not supported" % (args.gpus, world), file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__ as ge
+
+    lfmm = ge._import_package()
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(args.seed)
+    x = rng.random((args.n, 3))
+    q = rng.uniform(-1.0, 1.0, args.n)
+    xd = torch.from_numpy(x).to(dev)
+    qd = torch.from_numpy(q).to(dev)
+    pot = torch.empty(args.n, dtype=torch.float64, device=dev)
+    field = torch.empty((args.n, 3), dtype=torch.float64, device=dev)
+    ctx = lfmm.Context(0)
+
+    def step(timings=False):
+        return lfmm.fmm3d(xd, qd, terms=args.terms, tree_depth=args.depth, ctx=ctx, out=(pot, field),
+                          timings=timings)[2]
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print("bench: warmup done (n=%d terms=%d depth=%d)" % (args.n, args.terms, args.depth), file=sys.stderr,
+          flush=True)
+    reports = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        reports.append(step(timings=True))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = 1e3 * elapsed / args.steps
+    value = args.n / (elapsed / args.steps)
+    print("bench: timed %d steps, %.3f ms/step" % (args.steps, ms), file=sys.stderr, flush=True)
+
+    # exact op counts (after the timed region)
+    counts = lfmm.fmm3d(xd, qd, terms=args.terms, tree_depth=args.depth, ctx=ctx, out=(pot, field),
+                        timings=True, count_ops=True)[2]
+    m2l_ms = float(np.mean([r["t_m2l_kernels_ms"] for r in reports]))
+    launches = reports[0]["m2l_launches"]
+    leaf_ms = float(np.mean([r["t_leaf_kernel_ms"] for r in reports]))
+    flops = counts["m2l_ops"] * m2l_flops_per_op(args.terms)
+    achieved = flops / (m2l_ms * 1e-3) / 1e12
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "m2l_traffic.json")
+    if os.path.exists(tf):
+        try:
+            t = json.load(open(tf))
+            if t.get("n") == args.n and t.get("terms") == args.terms and t.get("tree_depth") == args.depth:
+                traffic = t.get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    roofline = {"kernel": "k_rtr<N,M2L> (M2L+L2L, one launch per level)", "bound": "mfma",
+                "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
+                "m2l_ops": counts["m2l_ops"], "flops_per_op": m2l_flops_per_op(args.terms),
+                "launches": launches, "avg_launch_ms": round(m2l_ms / max(launches, 1), 4),
+                "m2l_kernels_ms_per_step": round(m2l_ms, 4), "precision": "fp64 VALU"}
+    # near-field kernel as HBM-bound figure: 32 B per source read + 32 B per target in/out, per neighbour list
+    p2p_pairs = counts["p2p_pairs"]
+    leaf = {"kernel": "k_leaf<N> (P2P+L2P)", "ms_per_step": round(leaf_ms, 4), "p2p_pairs": p2p_pairs,
+            "gflops_p2p": round(20.0 * p2p_pairs / (leaf_ms * 1e-3) / 1e9, 1)}
+
+    accuracy = None
+    if not args.no_error:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        torch.cuda.synchronize()
+        idx = np.random.default_rng(args.seed + 1).choice(args.n, min(args.sample, args.n), replace=False)
+        p = pot.cpu().numpy()[idx]
+        f = field.cpu().numpy()[idx]
+        t1 = time.perf_counter()
+        dp, df = oracle.direct3d(x, q, tgt_index=idx, include_self=False)
+        print("bench: direct sum on %d targets took %.1f s" % (len(idx), time.perf_counter() - t1), file=sys.stderr,
+              flush=True)
+        phys = p - q[idx] * 1e4  # remove the reference's clamped self term q_i / 1e-4
+        ep = float(np.linalg.norm(phys - dp) / np.linalg.norm(dp))
+        ef = float(np.linalg.norm(f - df) / np.linalg.norm(df))
+        dpi = dp + q[idx] * 1e4
+        accuracy = {"targets": int(len(idx)), "rel_l2_potential": ep, "rel_l2_field": ef,
+                    "meets_1e-6": bool(ep <= 1e-6),
+                    "reference_convention": dict(zip(("max_err", "l2_err", "rel_err"),
+                                                     map(float, oracle.read_error(dpi, p))))}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        rs = np.random.default_rng(args.seed + 7)
+        xs = rs.random((args.cpu_n, 3))
+        qs = rs.uniform(-1.0, 1.0, args.cpu_n)
+        cdepth = max(2, args.depth - int(round(np.log(args.n / args.cpu_n) / np.log(8.0))))
+        t1 = time.perf_counter()
+        oracle.fmm3d(xs, qs, args.terms, cdepth)
+        tc = time.perf_counter() - t1
+        cpu = {"value": args.cpu_n / tc, "unit": "particles/s", "cores": oracle.num_threads(), "kind": "port",
+               "sample": "CPU oracle (C/OpenMP restatement of the reference CPU FMM) on %d uniform particles, "
+                         "%d terms, depth %d (same leaf density), one solve of %.1f s" % (args.cpu_n, args.terms,
+                                                                                         cdepth, tc)}
+        print("bench: cpu baseline %.1f s" % tc, file=sys.stderr, flush=True)
+
+    out = {"metric": METRIC, "value": value, "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "3D Laplace FMM, %d particles uniform in [0,1)^3, q~U[-1,1], p=%d (%d terms), "
+                                  "uniform octree depth %d (leaves at level %d)" % (args.n, args.terms - 1, args.terms,
+                                                                                     args.depth, args.depth - 1),
+                      "n": args.n, "terms": args.terms, "tree_depth": args.depth, "max_leaf": counts["max_leaf"],
+                      "parallelism": "single GPU"},
+           "accuracy": accuracy, "roofline": roofline, "leaf_kernel": leaf, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,159 @@
+/*
+ * lfmm.h -- C ABI of the MI355X-native Laplace FMM evaluation path.
+ *
+ * The reference exposes this path as C++ classes that compute on
+ * construction; there is no foreign-function layer in the reference, so the
+ * entry points below are exactly what a binding of those constructors needs:
+ *
+ *   lfmm_fmm3d  replaces fmm::gpu::FastMultipole3d<double>(particles, eps, N, tree_depth)
+ *               src/cuda/multipole3d.h:16 (and its CPU twin src/cpu/multipole3d.h:19-20),
+ *               and, with targets, the (sources, targets, ...) constructor
+ *               src/cuda/multipole3d.h:17 / src/cpu/multipole3d.h:21.
+ *               Outputs match the public members `potentials` and `forces`
+ *               (src/cuda/multipole3d.h:20-21): potential and
+ *               field = sum q_j (r_i - r_j)/|r_i - r_j|^3 in input order.
+ *
+ * The C++ drop-in classes in include/fmm/ wrap these calls with the
+ * reference's constructor signatures.
+ *
+ * Threading: one context per host thread (calls on one context serialise on
+ * its stream).  No C++ exceptions cross this boundary; every entry point
+ * returns an lfmm_status.
+ */
+#ifndef LFMM_H
+#define LFMM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same value as the reference's fmm::FMM_AUTO (src/common/defs.h:22). */
+#define LFMM_AUTO 2147483647
+
+/* Expansion-term caps of the reference (src/common/defs.h:31-32). */
+#define LFMM_MAX_TERMS_3D 18
+#define LFMM_MAX_TERMS_2D 30
+
+typedef enum lfmm_status {
+    LFMM_OK = 0,
+    LFMM_ERR_INVALID = 1,      /* null pointer, negative count, bad stride */
+    LFMM_ERR_TERMS = 2,        /* expansion terms outside [1, cap]         */
+    LFMM_ERR_DEPTH = 3,        /* tree depth outside the supported range   */
+    LFMM_ERR_NOMEM = 4,        /* device allocation failed                 */
+    LFMM_ERR_HIP = 5,          /* a HIP runtime call failed                */
+    LFMM_ERR_NONFINITE = 6,    /* NaN or Inf in positions or charges       */
+    LFMM_ERR_NODEVICE = 7,     /* no usable GPU                            */
+    LFMM_ERR_UNSUPPORTED = 8   /* option combination not implemented       */
+} lfmm_status;
+
+typedef enum lfmm_memory {
+    LFMM_MEM_HOST = 0,   /* all array arguments are host pointers   */
+    LFMM_MEM_DEVICE = 1  /* all array arguments are device pointers */
+} lfmm_memory;
+
+/* Flags (bitwise or). */
+#define LFMM_FLAG_EXCLUDE_SELF 1u  /* skip the i == j pair; the reference includes
+                                      q_i / FORCE_EPS in every potential
+                                      (src/common/simple_math.h:130-133) */
+#define LFMM_FLAG_TIMINGS 2u       /* fill the per-stage timings of the report
+                                      (adds stream synchronisations) */
+#define LFMM_FLAG_COUNT_OPS 4u     /* count M2L and near-field pair operations
+                                      into the report (extra kernels) */
+
+typedef struct lfmm_ctx lfmm_ctx;
+
+/* Strided particle view.  Particle i has position
+ * (pos[i*pos_stride], pos[i*pos_stride+1], pos[i*pos_stride+2]) and charge
+ * q[i*q_stride] (strides in doubles).  Packed SoA uses strides 3 and 1; the
+ * reference's 40-byte particle {Vector3d center; double q; size_t morton}
+ * (src/common/utils.h:26-32) uses pos = &p[0].center[0], q = &p[0].q and
+ * strides 5 and 5. */
+typedef struct lfmm_points3d {
+    int64_t n;
+    const double* pos;
+    int64_t pos_stride;
+    const double* q;      /* may be NULL for targets */
+    int64_t q_stride;
+} lfmm_points3d;
+
+typedef struct lfmm_report {
+    int32_t terms;          /* N: expansion terms used (order p = N - 1)  */
+    int32_t tree_depth;     /* number of levels; leaves at tree_depth - 1 */
+    int64_t num_leaves;
+    int64_t max_leaf;       /* largest leaf occupancy                     */
+    int64_t m2l_ops;        /* M2L translations performed (with LFMM_FLAG_COUNT_OPS, else -1) */
+    int64_t p2p_pairs;      /* near-field pair evaluations (with LFMM_FLAG_COUNT_OPS, else -1) */
+    double t_tree_ms, t_upward_ms, t_downward_ms, t_leaf_ms, t_total_ms;
+    /* With LFMM_FLAG_TIMINGS: summed device durations (HIP events on the
+     * solve's stream) of the M2L(+L2L) kernels, their launch count, and of
+     * the fused P2P+L2P leaf kernel. */
+    double t_m2l_kernels_ms;
+    int32_t m2l_launches;
+    int32_t reserved0;
+    double t_leaf_kernel_ms;
+} lfmm_report;
+
+/* Create a context bound to a GPU (device index as HIP numbers them). */
+int lfmm_create(lfmm_ctx** ctx, int device);
+void lfmm_destroy(lfmm_ctx* ctx);
+const char* lfmm_strerror(int status);
+/* Use a caller-owned hipStream_t (NULL restores the context's own stream). */
+int lfmm_set_stream(lfmm_ctx* ctx, void* hip_stream);
+/* Release cached device workspace (it is otherwise reused across calls). */
+int lfmm_trim(lfmm_ctx* ctx);
+
+/*
+ * 3D Laplace FMM.  targets == NULL (or targets->n == 0) is the single-set
+ * solve: every source is also a target and outputs are per source.
+ *   eps, terms, tree_depth: the reference's constructor arguments; terms
+ *     and tree_depth accept LFMM_AUTO and then follow the reference's
+ *     formulas (src/cpu/multipole3d.cpp:1328-1337).  terms is clamped to 18.
+ *   leaf_max > 0 (used only when tree_depth == LFMM_AUTO): pick the smallest
+ *     uniform depth whose fullest leaf holds at most leaf_max particles.
+ *   potential: n_out doubles; field: n_out*3 doubles (x,y,z per particle);
+ *     either may be NULL.  n_out = targets ? targets->n : sources->n.
+ *   report: optional.
+ * Single-set solves are implemented; target sets return
+ * LFMM_ERR_UNSUPPORTED in this version.
+ */
+int lfmm_fmm3d(lfmm_ctx* ctx, const lfmm_points3d* sources, const lfmm_points3d* targets,
+               double eps, int terms, int tree_depth, int leaf_max,
+               int memory, unsigned flags,
+               double* potential, double* field, lfmm_report* report);
+
+/*
+ * The Morton octree of a particle set, as built inside lfmm_fmm3d.  Replaces
+ * the reference's MortonTree<3, point3d, double> (src/cpu/morton_tree.h:24-45,
+ * GPU twin src/cuda/morton_tree.h): positions_map, level_sizes and the cell
+ * keys / ranges of every level.  Inputs are host pointers (charges unused);
+ * outputs are host arrays, any of which may be NULL:
+ *   depth_out      number of levels (leaves at depth-1)
+ *   level_sizes    [32] boxes per level
+ *   geom           [4] {shift_x, shift_y, shift_z, scale} (morton_tree.cpp:293-301)
+ *   pos_map        [n] input index -> sorted position (stable key order)
+ *   keys           [sum level_sizes] Morton keys, level 0 first
+ *   begins         [sum (level_sizes+1)] per level: the start of each box's
+ *                  range (particles at the leaves, children above) plus the end
+ */
+int lfmm_tree3d(lfmm_ctx* ctx, const lfmm_points3d* particles, int tree_depth, int leaf_max,
+                int32_t* depth_out, int64_t* level_sizes, double* geom, int64_t* pos_map,
+                uint64_t* keys, int64_t* begins);
+
+/* Inspection hooks of the host FP64 table builder (no reference
+ * counterpart; used by the CPU test suite to pin the tables without a GPU).
+ * lfmm_host_wigner_d: standard small Wigner matrix d^j(beta), row-major
+ * [m'+j][m+j].  lfmm_host_op_tables: copies table `which` (0 rot, 1 ephi,
+ * 2 trans, 3 dir_theta, 4 dir_trans, 5 dir_vec, 6 class_dirs, 7 knm) for N
+ * terms into out (if capacity suffices) and returns its element count, or a
+ * negative status. */
+int lfmm_host_wigner_d(int j, double beta, double* out);
+int64_t lfmm_host_op_tables(int N, int which, void* out, int64_t capacity);
+
+/* Library version string. */
+const char* lfmm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,261 @@
+"""Python binding of the MI355X Laplace FMM C ABI (include/lfmm.h).
+
+The product is the HIP library laplace-fmm_amd/_build/liblfmm.so; this module
+only marshals arguments to it through ctypes (host numpy arrays, or device
+buffers given as torch tensors).  It mirrors the reference's drop-in class
+fmm::gpu::FastMultipole3d<double> (src/cuda/multipole3d.h:7-42): construct
+with particles, eps, N, tree_depth and read `.potentials` and `.forces`.
+
+There is no CPU fallback: if the library is missing or no GPU is present the
+calls raise.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liblfmm.so")
+CSRC = os.path.join(_HERE, "csrc")
+
+FMM_AUTO = 2147483647
+MAX_TERMS_3D = 18
+MEM_HOST, MEM_DEVICE = 0, 1
+FLAG_EXCLUDE_SELF, FLAG_TIMINGS, FLAG_COUNT_OPS = 1, 2, 4
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "expansion terms out of range", 3: "tree depth out of range",
+          4: "out of device memory", 5: "HIP runtime error", 6: "non-finite input", 7: "no usable GPU",
+          8: "unsupported option"}
+
+# Every extern "C" entry point declared in include/lfmm.h.
+EXPORTS = ("lfmm_create", "lfmm_destroy", "lfmm_strerror", "lfmm_set_stream", "lfmm_trim", "lfmm_fmm3d",
+           "lfmm_tree3d", "lfmm_version", "lfmm_host_wigner_d", "lfmm_host_op_tables")
+
+
+class LfmmError(RuntimeError):
+    def __init__(self, code, where=""):
+        self.code = code
+        super().__init__("%s: lfmm status %d (%s)" % (where, code, STATUS.get(code, "?")))
+
+
+class Points3d(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("pos", ctypes.c_void_p), ("pos_stride", ctypes.c_int64),
+                ("q", ctypes.c_void_p), ("q_stride", ctypes.c_int64)]
+
+
+class Report(ctypes.Structure):
+    _fields_ = [("terms", ctypes.c_int32), ("tree_depth", ctypes.c_int32), ("num_leaves", ctypes.c_int64),
+                ("max_leaf", ctypes.c_int64), ("m2l_ops", ctypes.c_int64), ("p2p_pairs", ctypes.c_int64),
+                ("t_tree_ms", ctypes.c_double), ("t_upward_ms", ctypes.c_double),
+                ("t_downward_ms", ctypes.c_double), ("t_leaf_ms", ctypes.c_double),
+                ("t_total_ms", ctypes.c_double), ("t_m2l_kernels_ms", ctypes.c_double),
+                ("m2l_launches", ctypes.c_int32), ("reserved0", ctypes.c_int32), ("t_leaf_kernel_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def build(jobs=8, force=False):
+    """Compile liblfmm.so for gfx950 with hipcc (works without a GPU)."""
+    cmd = ["make", "-C", CSRC, "-j%d" % jobs]
+    if force:
+        cmd.append("-B")
+    subprocess.run(cmd, check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError("liblfmm.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        L.lfmm_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
+        L.lfmm_create.restype = ctypes.c_int
+        L.lfmm_destroy.argtypes = [ctypes.c_void_p]
+        L.lfmm_destroy.restype = None
+        L.lfmm_strerror.argtypes = [ctypes.c_int]
+        L.lfmm_strerror.restype = ctypes.c_char_p
+        L.lfmm_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.lfmm_trim.argtypes = [ctypes.c_void_p]
+        L.lfmm_fmm3d.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points3d), ctypes.POINTER(Points3d),
+                                 ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Report)]
+        L.lfmm_fmm3d.restype = ctypes.c_int
+        L.lfmm_tree3d.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points3d), ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_void_p]
+        L.lfmm_tree3d.restype = ctypes.c_int
+        L.lfmm_version.restype = ctypes.c_char_p
+        L.lfmm_host_wigner_d.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
+        L.lfmm_host_wigner_d.restype = ctypes.c_int
+        L.lfmm_host_op_tables.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64]
+        L.lfmm_host_op_tables.restype = ctypes.c_int64
+        _lib = L
+    return _lib
+
+
+def _check(rc, where):
+    if rc != 0:
+        raise LfmmError(rc, where)
+
+
+class Context:
+    """One lfmm_ctx (a HIP stream plus cached workspace) on one GPU."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        _check(lib().lfmm_create(ctypes.byref(self._h), int(device)), "lfmm_create")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, stream_ptr):
+        _check(lib().lfmm_set_stream(self._h, ctypes.c_void_p(stream_ptr)), "lfmm_set_stream")
+
+    def close(self):
+        if self._h:
+            lib().lfmm_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _is_torch_cuda(x):
+    return hasattr(x, "is_cuda") and x.is_cuda
+
+
+def fmm3d(xyz, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exclude_self=False,
+          timings=False, count_ops=False, ctx=None, out=None):
+    """Single-set 3D FMM: potential (n,) and field (n,3) at every source.
+
+    xyz (n,3) and q (n,) are float64 numpy arrays (host path) or CUDA torch
+    tensors (device path; outputs are then CUDA tensors, or written into
+    `out=(pot, field)`).  Returns (pot, field, report_dict).
+    """
+    ctx = ctx or default_context()
+    flags = ((FLAG_EXCLUDE_SELF if exclude_self else 0) | (FLAG_TIMINGS if timings else 0) |
+             (FLAG_COUNT_OPS if count_ops else 0))
+    rep = Report()
+    if _is_torch_cuda(xyz):
+        import torch
+        if xyz.dtype != torch.float64 or q.dtype != torch.float64:
+            raise TypeError("device inputs must be float64")
+        if xyz.stride(-1) != 1 or q.stride(0) < 1:
+            raise ValueError("positions must be contiguous in the last dimension")
+        n = xyz.shape[0]
+        pts = Points3d(n, xyz.data_ptr(), xyz.stride(0), q.data_ptr(), q.stride(0))
+        if out is None:
+            pot = torch.empty(n, dtype=torch.float64, device=xyz.device)
+            field = torch.empty((n, 3), dtype=torch.float64, device=xyz.device)
+        else:
+            pot, field = out
+        rc = lib().lfmm_fmm3d(ctx.handle, ctypes.byref(pts), None, float(eps), int(terms), int(tree_depth),
+                              int(leaf_max), MEM_DEVICE, flags, ctypes.c_void_p(pot.data_ptr()),
+                              ctypes.c_void_p(field.data_ptr()), ctypes.byref(rep))
+        _check(rc, "lfmm_fmm3d")
+        return pot, field, rep.as_dict()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    n = len(q)
+    pot = np.empty(n)
+    field = np.empty((n, 3))
+    pts = Points3d(n, xyz.ctypes.data, 3, q.ctypes.data, 1)
+    rc = lib().lfmm_fmm3d(ctx.handle, ctypes.byref(pts), None, float(eps), int(terms), int(tree_depth),
+                          int(leaf_max), MEM_HOST, flags, pot.ctypes.data, field.ctypes.data, ctypes.byref(rep))
+    _check(rc, "lfmm_fmm3d")
+    return pot, field, rep.as_dict()
+
+
+def tree3d(xyz, q, tree_depth, leaf_max=0, ctx=None):
+    """The GPU Morton tree: dict with depth, level sizes, geometry (shift, scale),
+    pos_map (input -> sorted position), per-level keys and ranges."""
+    ctx = ctx or default_context()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    n = len(q)
+    pts = Points3d(n, xyz.ctypes.data, 3, q.ctypes.data, 1)
+    sizes = np.zeros(32, dtype=np.int64)
+    geom = np.zeros(4)
+    pos_map = np.zeros(n, dtype=np.int64)
+    depth = np.zeros(1, dtype=np.int32)
+    # first call sizes the key/range buffers: total boxes <= n * depth
+    cap = n * (max(int(tree_depth), 1) if tree_depth != FMM_AUTO else 17) + 64
+    keys = np.zeros(cap, dtype=np.uint64)
+    ranges = np.zeros(cap + 64, dtype=np.int64)
+    rc = lib().lfmm_tree3d(ctx.handle, ctypes.byref(pts), int(tree_depth), int(leaf_max), depth.ctypes.data,
+                           sizes.ctypes.data, geom.ctypes.data, pos_map.ctypes.data, keys.ctypes.data,
+                           ranges.ctypes.data)
+    _check(rc, "lfmm_tree3d")
+    d = int(depth[0])
+    out = {"depth": d, "level_sizes": sizes[:d].copy(), "shift": geom[:3].copy(), "scale": float(geom[3]),
+           "pos_map": pos_map, "keys": [], "begin": []}
+    o = b = 0
+    for l in range(d):
+        s = int(sizes[l])
+        out["keys"].append(keys[o:o + s].copy())
+        out["begin"].append(ranges[b:b + s + 1].copy())
+        o += s
+        b += s + 1
+    return out
+
+
+def host_wigner_d(j, beta):
+    """Host table builder's small Wigner d^j(beta), [(2j+1) x (2j+1)] indexed [m'+j, m+j]."""
+    out = np.zeros((2 * j + 1) * (2 * j + 1))
+    _check(lib().lfmm_host_wigner_d(int(j), float(beta), out.ctypes.data), "lfmm_host_wigner_d")
+    return out.reshape(2 * j + 1, 2 * j + 1)
+
+
+HOST_TABLES = {"rot": 0, "ephi": 1, "trans": 2, "dir_theta": 3, "dir_trans": 4, "dir_vec": 5, "class_dirs": 6,
+               "knm": 7}
+
+
+def host_op_tables(N, which):
+    """Copy of one host-built operator table for N terms (float64 or int32 numpy array)."""
+    w = HOST_TABLES[which]
+    count = lib().lfmm_host_op_tables(int(N), w, None, 0)
+    if count < 0:
+        raise LfmmError(int(-count), "lfmm_host_op_tables")
+    isint = which in ("dir_theta", "dir_trans", "dir_vec", "class_dirs")
+    out = np.zeros(count, dtype=np.int32 if isint else np.float64)
+    got = lib().lfmm_host_op_tables(int(N), w, out.ctypes.data, count)
+    if got != count:
+        raise LfmmError(1, "lfmm_host_op_tables")
+    return out
+
+
+class FastMultipole3d:
+    """Drop-in for fmm::gpu::FastMultipole3d<double> (src/cuda/multipole3d.h:7-42).
+
+    particles: (n,4) array [x, y, z, q] (or a tuple (xyz, q)).  Computes on
+    construction; results in `.potentials` (n,) and `.forces` (n,3), in input
+    order.  eps / N / tree_depth follow the reference's FMM_AUTO rules.
+    """
+
+    def __init__(self, particles, eps=1e-8, N=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, ctx=None):
+        if isinstance(particles, tuple):
+            xyz, q = particles
+        else:
+            p = np.asarray(particles, dtype=np.float64)
+            xyz, q = p[:, :3], p[:, 3]
+        self.potentials, self.forces, self.report = fmm3d(xyz, q, eps=eps, terms=N, tree_depth=tree_depth,
+                                                          leaf_max=leaf_max, ctx=ctx)

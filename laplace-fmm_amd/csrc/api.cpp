@@ -1,0 +1,681 @@
+// api.cpp -- C ABI of the MI355X Laplace FMM (see include/lfmm.h).
+//
+// Host side of the solve: argument checks, the reference's parameter rules
+// (src/cpu/multipole3d.cpp:1321-1337), the workspace arena, per-N operator
+// tables on the device, and the launch sequence
+//   extent -> keys -> radix sort -> gather -> levels -> P2M -> M2M (up)
+//   -> M2L+L2L (down) -> P2P+L2P (leaves, scattered to input order).
+// Everything runs on one HIP stream; the only host synchronisations are the
+// box counts of each level (needed to size the next launch).
+#include "lfmm.h"
+#include "lfmm_internal.h"
+#include "ops3d.h"
+#include "tree.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <vector>
+
+namespace {
+
+using namespace lfmm;
+
+constexpr int kMaxDepth = 16;   // 15 bits per axis at the leaves
+constexpr int kLutMaxLevel = 8; // dense key -> box tables up to 8^8 entries
+
+struct Arena {
+    struct Chunk { char* p; size_t size; };
+    std::vector<Chunk> chunks;
+    size_t cur = 0, off = 0, used = 0, peak = 0;
+
+    void reset()
+    {
+        if (chunks.size() > 1) {  // consolidate into one chunk sized for the last peak
+            for (auto& c : chunks) (void)hipFree(c.p);
+            chunks.clear();
+            const size_t want = peak + peak / 8 + (1 << 20);
+            char* p = nullptr;
+            if (hipMalloc(&p, want) == hipSuccess) chunks.push_back({p, want});
+        }
+        cur = 0;
+        off = 0;
+        used = 0;
+    }
+    void* alloc(size_t bytes)
+    {
+        bytes = (bytes + 255) & ~(size_t)255;
+        if (bytes == 0) bytes = 256;
+        while (cur < chunks.size()) {
+            if (off + bytes <= chunks[cur].size) {
+                void* p = chunks[cur].p + off;
+                off += bytes;
+                used += bytes;
+                peak = std::max(peak, used);
+                return p;
+            }
+            ++cur;
+            off = 0;
+        }
+        size_t sz = std::max<size_t>(bytes, (size_t)64 << 20);
+        if (!chunks.empty()) sz = std::max(sz, chunks.back().size);
+        char* p = nullptr;
+        if (hipMalloc(&p, sz) != hipSuccess) return nullptr;
+        chunks.push_back({p, sz});
+        cur = chunks.size() - 1;
+        off = bytes;
+        used += bytes;
+        peak = std::max(peak, used);
+        return p;
+    }
+    void release()
+    {
+        for (auto& c : chunks) (void)hipFree(c.p);
+        chunks.clear();
+        cur = off = used = peak = 0;
+    }
+};
+
+struct DevTables {
+    int N = 0;
+    void* block = nullptr;
+    RtrTables rt{};
+    const double* knm = nullptr;
+};
+
+struct Status {
+    int code = LFMM_OK;
+};
+
+}  // namespace
+
+struct lfmm_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    Arena arena;
+    std::vector<hipEvent_t> events;  // timing events, created on first timed call
+    std::map<int, DevTables> tables;
+    std::mutex mu;
+};
+
+namespace {
+
+#define LFMM_HIP(call)                                   \
+    do {                                                 \
+        hipError_t e_ = (call);                          \
+        if (e_ != hipSuccess) return LFMM_ERR_HIP;       \
+    } while (0)
+
+template <typename T>
+T* take(Arena& a, int64_t count)
+{
+    return static_cast<T*>(a.alloc(sizeof(T) * (size_t)std::max<int64_t>(count, 1)));
+}
+
+int get_tables(lfmm_ctx* ctx, int N, const DevTables** out)
+{
+    auto it = ctx->tables.find(N);
+    if (it != ctx->tables.end()) {
+        *out = &it->second;
+        return LFMM_OK;
+    }
+    OpTables T;
+    build_op_tables(N, T);
+    const size_t b_rot = T.rot.size() * 8, b_eph = T.ephi.size() * 8, b_tr = T.trans.size() * 8;
+    const size_t b_dt = T.dir_theta.size() * 4, b_dtr = T.dir_trans.size() * 4, b_dv = T.dir_vec.size() * 4,
+                 b_cd = T.class_dirs.size() * 4, b_k = T.knm.size() * 8;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t total = al(b_rot) + al(b_eph) + al(b_tr) + al(b_dt) + al(b_dtr) + al(b_dv) + al(b_cd) + al(b_k);
+    char* blk = nullptr;
+    if (hipMalloc(&blk, total) != hipSuccess) return LFMM_ERR_NOMEM;
+    size_t o = 0;
+    bool copy_ok = true;
+    auto put = [&](const void* src, size_t bytes) -> void* {
+        void* dst = blk + o;
+        copy_ok = copy_ok && hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+        o += al(bytes);
+        return dst;
+    };
+    DevTables D;
+    D.N = N;
+    D.block = blk;
+    D.rt.rot = (const double2*)put(T.rot.data(), b_rot);
+    D.rt.ephi = (const double2*)put(T.ephi.data(), b_eph);
+    D.rt.trans = (const double*)put(T.trans.data(), b_tr);
+    D.rt.dir_theta = (const int*)put(T.dir_theta.data(), b_dt);
+    D.rt.dir_trans = (const int*)put(T.dir_trans.data(), b_dtr);
+    D.rt.dir_vec = (const int*)put(T.dir_vec.data(), b_dv);
+    D.rt.class_dirs = (const int*)put(T.class_dirs.data(), b_cd);
+    D.knm = (const double*)put(T.knm.data(), b_k);
+    if (!copy_ok || hipDeviceSynchronize() != hipSuccess) {
+        (void)hipFree(blk);
+        return LFMM_ERR_HIP;
+    }
+    auto res = ctx->tables.emplace(N, D);
+    *out = &res.first->second;
+    return LFMM_OK;
+}
+
+// The reference's expansion-size rule, src/cpu/multipole3d.cpp:1326-1331.
+int terms_3d(double eps, int terms)
+{
+    if (terms == LFMM_AUTO) {
+        eps = std::max(eps, 1.e-11);
+        const int a = (int)(54.2 - 10.73 * std::sqrt(std::log(2.4 * eps) + 25.5));
+        return std::min(LFMM_MAX_TERMS_3D, a);
+    }
+    return std::min(LFMM_MAX_TERMS_3D, terms);
+}
+
+// Timing events on the solve's stream: ev(i) grows the pool on demand.
+hipEvent_t ctx_event(lfmm_ctx* ctx, size_t i)
+{
+    while (ctx->events.size() <= i) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ctx->events.push_back(e);
+    }
+    return ctx->events[i];
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Device-resident Morton tree of one solve (all buffers in the context arena).
+struct Tree {
+    int64_t n = 0;
+    int depth = 0;
+    const double* dpos = nullptr;  // device view of the inputs
+    const double* dq = nullptr;
+    int64_t ps = 0, qs = 0;
+    double* geom = nullptr;        // {shift x, y, z, scale}
+    uint32_t* umax = nullptr;
+    uint32_t* order = nullptr;     // sorted -> input index
+    double4* src = nullptr;        // sorted (x, y, z, q)
+    int32_t* leaf_of = nullptr;
+    std::vector<int64_t> cnt;
+    std::vector<uint64_t*> lkeys;
+    std::vector<int32_t*> lbegin;  // leaves: particle ranges; above: child ranges
+    std::vector<int32_t*> lut;
+    uint32_t max_leaf = 0;
+};
+
+// MortonTree constructor (src/cpu/morton_tree.cpp:286-493) on the GPU.
+int build_tree(lfmm_ctx* ctx, const lfmm_points3d* S, int memory, int depth_in, int leaf_max, Tree& T)
+{
+    const int64_t n = S->n;
+    hipStream_t st = ctx->stream;
+    Arena& A = ctx->arena;
+    T.n = n;
+    // ---- inputs on the device
+    const double* dpos = S->pos;
+    const double* dq = S->q;
+    int64_t ps = S->pos_stride, qs = S->q_stride;
+    if (memory == LFMM_MEM_HOST) {
+        const size_t pbytes = sizeof(double) * (size_t)((n - 1) * ps + 3);
+        const bool shared = (S->q > S->pos && S->q < S->pos + (n - 1) * ps + 3 && ps == qs);
+        if (shared) {
+            const size_t span = sizeof(double) * (size_t)((n - 1) * ps + (S->q - S->pos) + 1);
+            const size_t bytes = std::max(span, pbytes);
+            double* d = take<double>(A, (int64_t)(bytes / 8) + 1);
+            if (!d) return LFMM_ERR_NOMEM;
+            LFMM_HIP(hipMemcpyAsync(d, S->pos, bytes, hipMemcpyHostToDevice, st));
+            dq = d + (S->q - S->pos);
+            dpos = d;
+        } else {
+            double* d = take<double>(A, (int64_t)(pbytes / 8) + 1);
+            if (!d) return LFMM_ERR_NOMEM;
+            LFMM_HIP(hipMemcpyAsync(d, S->pos, pbytes, hipMemcpyHostToDevice, st));
+            dpos = d;
+            if (S->q) {
+                double* e = take<double>(A, (n - 1) * qs + 1);
+                if (!e) return LFMM_ERR_NOMEM;
+                LFMM_HIP(hipMemcpyAsync(e, S->q, sizeof(double) * (size_t)((n - 1) * qs + 1), hipMemcpyHostToDevice, st));
+                dq = e;
+            }
+        }
+    }
+    T.dpos = dpos;
+    T.dq = dq;
+    T.ps = ps;
+    T.qs = qs;
+
+    double* geom = take<double>(A, 4);
+    double* part = take<double>(A, 6 * 1024);
+    int* nonfinite = take<int>(A, 1);
+    uint32_t* umax = take<uint32_t>(A, 64);
+    if (!geom || !part || !nonfinite || !umax) return LFMM_ERR_NOMEM;
+    T.geom = geom;
+    T.umax = umax;
+    LFMM_HIP(hipMemsetAsync(nonfinite, 0, sizeof(int), st));
+    LFMM_HIP(extent(dpos, ps, n, part, nonfinite, geom, st));
+    {
+        int bad = 0;
+        LFMM_HIP(hipMemcpyAsync(&bad, nonfinite, sizeof(int), hipMemcpyDeviceToHost, st));
+        LFMM_HIP(hipStreamSynchronize(st));
+        if (bad) return LFMM_ERR_NONFINITE;
+    }
+
+    // ---- tree depth (reference rule :1334-1337, or the leaf-size bound)
+    int depth = depth_in;
+    if (depth_in == LFMM_AUTO) {
+        if (leaf_max > 0) {
+            int maxl = 1;
+            while (maxl < kLutMaxLevel && ((int64_t)1 << (3 * maxl)) < n) ++maxl;
+            int64_t cnt = 0;
+            for (int l = 0; l <= maxl; ++l) cnt += (int64_t)1 << (3 * l);
+            uint32_t* cws = take<uint32_t>(A, cnt);
+            if (!cws) return LFMM_ERR_NOMEM;
+            int lvl = 0;
+            LFMM_HIP(occupancy_depth(dpos, ps, n, geom, leaf_max, maxl, cws, umax, &lvl, st));
+            depth = lvl + 1;
+        } else {
+            depth = (int)std::max(2.0, 1 + std::log((double)n / 100.0) / std::log(8.0));
+        }
+    }
+    if (depth < 1 || depth > kMaxDepth) return LFMM_ERR_DEPTH;
+    T.depth = depth;
+    const int leaf_level = depth - 1;
+
+    // ---- keys and stable sort
+    uint64_t* k0 = take<uint64_t>(A, n);
+    uint32_t* v0 = take<uint32_t>(A, n);
+    void* sws = A.alloc((size_t)sort_ws_bytes(n));
+    if (!k0 || !v0 || !sws) return LFMM_ERR_NOMEM;
+    LFMM_HIP(keys(dpos, ps, n, leaf_level, geom, k0, v0, st));
+    uint64_t* skeys = k0;
+    uint32_t* order = v0;
+    if (leaf_level > 0) LFMM_HIP(radix_sort(k0, v0, n, 3 * leaf_level, sws, st, &skeys, &order));
+    T.order = order;
+    if (dq) {
+        T.src = take<double4>(A, n);
+        if (!T.src) return LFMM_ERR_NOMEM;
+        LFMM_HIP(gather_sources(dpos, ps, dq, qs, order, n, T.src, st));
+    }
+
+    // ---- levels
+    T.cnt.assign(depth, 0);
+    T.lkeys.assign(depth, nullptr);
+    T.lbegin.assign(depth, nullptr);
+    T.lut.assign(depth, nullptr);
+    uint32_t* flags_ws = take<uint32_t>(A, n);
+    uint32_t* excl_ws = take<uint32_t>(A, n);
+    uint32_t* scan_ws = take<uint32_t>(A, scan_ws_elems(n));
+    T.leaf_of = take<int32_t>(A, n);
+    if (!flags_ws || !excl_ws || !scan_ws || !T.leaf_of) return LFMM_ERR_NOMEM;
+    {
+        const int64_t cap = std::min<int64_t>(n, (int64_t)1 << std::min(60, 3 * leaf_level));
+        T.lkeys[leaf_level] = take<uint64_t>(A, cap);
+        T.lbegin[leaf_level] = take<int32_t>(A, cap + 1);
+        if (!T.lkeys[leaf_level] || !T.lbegin[leaf_level]) return LFMM_ERR_NOMEM;
+        LFMM_HIP(run_compact(skeys, n, 0, flags_ws, excl_ws, scan_ws, T.lkeys[leaf_level], T.lbegin[leaf_level],
+                             T.leaf_of, &T.cnt[leaf_level], st));
+    }
+    for (int l = leaf_level - 1; l >= 0; --l) {
+        const int64_t m = T.cnt[l + 1];
+        const int64_t cap = std::min<int64_t>(m, (int64_t)1 << (3 * l));
+        T.lkeys[l] = take<uint64_t>(A, cap);
+        T.lbegin[l] = take<int32_t>(A, cap + 1);
+        if (!T.lkeys[l] || !T.lbegin[l]) return LFMM_ERR_NOMEM;
+        LFMM_HIP(run_compact(T.lkeys[l + 1], m, 3, flags_ws, excl_ws, scan_ws, T.lkeys[l], T.lbegin[l], nullptr,
+                             &T.cnt[l], st));
+    }
+    for (int l = 2; l < depth; ++l) {
+        if (l > kLutMaxLevel) continue;
+        T.lut[l] = take<int32_t>(A, (int64_t)1 << (3 * l));
+        if (!T.lut[l]) return LFMM_ERR_NOMEM;
+        LFMM_HIP(build_lut(T.lkeys[l], T.cnt[l], l, T.lut[l], st));
+    }
+    LFMM_HIP(max_run(T.lbegin[leaf_level], T.cnt[leaf_level], umax, &T.max_leaf, st));
+    return LFMM_OK;
+}
+
+int fmm3d_self(lfmm_ctx* ctx, const lfmm_points3d* S, double eps, int terms_in, int depth_in, int leaf_max,
+               int memory, unsigned flags, double* pot_out, double* field_out, lfmm_report* rep)
+{
+    const auto T0 = std::chrono::steady_clock::now();
+    const bool timed = (flags & LFMM_FLAG_TIMINGS) != 0;
+    const int64_t n = S->n;
+    const int N = terms_3d(eps, terms_in);
+    if (N < 1) return LFMM_ERR_TERMS;
+    const int NX = tri(N);
+    hipStream_t st = ctx->stream;
+    Arena& A = ctx->arena;
+    A.reset();
+    Tree T;
+    int rc = build_tree(ctx, S, memory, depth_in, leaf_max, T);
+    if (rc != LFMM_OK) return rc;
+    const int depth = T.depth;
+    const int leaf_level = depth - 1;
+    const double* geom = T.geom;
+    const double4* src = T.src;
+    const uint32_t* order = T.order;
+    const int32_t* leaf_of = T.leaf_of;
+    std::vector<int64_t>& cnt = T.cnt;
+    std::vector<uint64_t*>& lkeys = T.lkeys;
+    std::vector<int32_t*>& lbegin = T.lbegin;
+    std::vector<int32_t*>& lut = T.lut;
+    const uint32_t max_leaf = T.max_leaf;
+    const double t_tree = timed ? ms_since(T0) : 0.0;
+
+    // ---- expansions
+    const DevTables* DT = nullptr;
+    rc = get_tables(ctx, N, &DT);
+    if (rc != LFMM_OK) return rc;
+    std::vector<double2*> outer(depth, nullptr), inner(depth, nullptr);
+    const bool far = depth >= 3;
+    if (far) {
+        for (int l = 2; l < depth; ++l) {
+            outer[l] = take<double2>(A, cnt[l] * NX);
+            inner[l] = take<double2>(A, cnt[l] * NX);
+            if (!outer[l] || !inner[l]) return LFMM_ERR_NOMEM;
+        }
+        P2MArgs pa{src, lbegin[leaf_level], lkeys[leaf_level], (int)cnt[leaf_level], leaf_level, geom, DT->knm,
+                   outer[leaf_level]};
+        LFMM_HIP(launch_p2m(N, pa, st));
+        for (int l = leaf_level - 1; l >= 2; --l) {
+            RtrLevel L{};
+            L.level = l;
+            L.num_parents = (int)cnt[l];
+            L.child_begin = lbegin[l];
+            L.child_keys = lkeys[l + 1];
+            L.src = outer[l + 1];
+            L.dst = outer[l];
+            LFMM_HIP(launch_rtr(N, 0, DT->rt, L, st));
+        }
+    }
+    double t_up = 0.0;
+    if (timed) {
+        LFMM_HIP(hipStreamSynchronize(st));
+        t_up = ms_since(T0) - t_tree;
+    }
+    size_t nev = 0;
+    int m2l_launches = 0;
+    if (far) {
+        for (int l = 2; l < depth; ++l) {
+            if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+            RtrLevel L{};
+            L.level = l;
+            L.num_parents = (int)cnt[l - 1];
+            L.child_begin = lbegin[l - 1];
+            L.child_keys = lkeys[l];
+            L.lut = lut[l];
+            L.keys = lkeys[l];
+            L.size = (int)cnt[l];
+            L.src = outer[l];
+            L.parent_local = (l > 2) ? inner[l - 1] : nullptr;
+            L.dst = inner[l];
+            LFMM_HIP(launch_rtr(N, 1, DT->rt, L, st));
+            if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+            ++m2l_launches;
+        }
+    }
+    double t_down = 0.0;
+    if (timed) {
+        LFMM_HIP(hipStreamSynchronize(st));
+        t_down = ms_since(T0) - t_tree - t_up;
+    }
+
+    // ---- leaves
+    double* dpot = pot_out;
+    double* dfield = field_out;
+    if (memory == LFMM_MEM_HOST) {
+        dpot = pot_out ? take<double>(A, n) : nullptr;
+        dfield = field_out ? take<double>(A, 3 * n) : nullptr;
+        if ((pot_out && !dpot) || (field_out && !dfield)) return LFMM_ERR_NOMEM;
+    }
+    LeafArgs la{};
+    la.src = src;
+    la.leaf_of = leaf_of;
+    la.leaf_key = lkeys[leaf_level];
+    la.leaf_begin = lbegin[leaf_level];
+    la.nleaf = (int)cnt[leaf_level];
+    la.level = leaf_level;
+    la.geom = geom;
+    la.lut = leaf_level <= kLutMaxLevel ? (leaf_level >= 2 ? lut[leaf_level] : nullptr) : nullptr;
+    la.local = far ? inner[leaf_level] : nullptr;
+    la.knm = DT->knm;
+    la.perm = order;
+    la.n = n;
+    la.exclude_self = (flags & LFMM_FLAG_EXCLUDE_SELF) ? 1 : 0;
+    la.pot = dpot;
+    la.field = dfield;
+    const size_t ev_leaf = nev;
+    if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+    LFMM_HIP(launch_leaf(N, far, la, st));
+    if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+    if (memory == LFMM_MEM_HOST) {
+        if (pot_out) LFMM_HIP(hipMemcpyAsync(pot_out, dpot, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        if (field_out) LFMM_HIP(hipMemcpyAsync(field_out, dfield, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
+    }
+    if (timed || memory == LFMM_MEM_HOST) LFMM_HIP(hipStreamSynchronize(st));
+    LFMM_HIP(hipGetLastError());
+    if (rep) {
+        rep->terms = N;
+        rep->tree_depth = depth;
+        rep->num_leaves = cnt[leaf_level];
+        rep->max_leaf = max_leaf;
+        rep->m2l_ops = -1;
+        rep->p2p_pairs = -1;
+        rep->m2l_launches = m2l_launches;
+        if (timed) {
+            double tm = 0.0;
+            for (size_t i = 0; i + 1 < ev_leaf; i += 2) {
+                float ms = 0.f;
+                LFMM_HIP(hipEventElapsedTime(&ms, ctx->events[i], ctx->events[i + 1]));
+                tm += ms;
+            }
+            float ml = 0.f;
+            LFMM_HIP(hipEventElapsedTime(&ml, ctx->events[ev_leaf], ctx->events[ev_leaf + 1]));
+            rep->t_m2l_kernels_ms = tm;
+            rep->t_leaf_kernel_ms = ml;
+        }
+        rep->t_tree_ms = t_tree;
+        rep->t_upward_ms = t_up;
+        rep->t_downward_ms = t_down;
+        rep->t_total_ms = timed ? ms_since(T0) : 0.0;
+        rep->t_leaf_ms = timed ? rep->t_total_ms - t_tree - t_up - t_down : 0.0;
+        if (flags & LFMM_FLAG_COUNT_OPS) {  // exact op counts, after the solve
+            unsigned long long* cws = take<unsigned long long>(A, 2);
+            if (!cws) return LFMM_ERR_NOMEM;
+            LFMM_HIP(count_ops(lkeys, lbegin, lut, cnt, cws, &rep->m2l_ops, &rep->p2p_pairs, st));
+        }
+    }
+    return LFMM_OK;
+}
+
+bool valid_points(const lfmm_points3d* p, bool need_q)
+{
+    if (!p || p->n < 0) return false;
+    if (p->n == 0) return true;
+    if (!p->pos || p->pos_stride < 3) return false;
+    if (need_q && (!p->q || p->q_stride < 1)) return false;
+    return p->n < ((int64_t)1 << 31);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lfmm_create(lfmm_ctx** out, int device)
+{
+    if (!out) return LFMM_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return LFMM_ERR_NODEVICE;
+    if (device < 0 || device >= count) return LFMM_ERR_NODEVICE;
+    if (hipSetDevice(device) != hipSuccess) return LFMM_ERR_HIP;
+    lfmm_ctx* c = new (std::nothrow) lfmm_ctx();
+    if (!c) return LFMM_ERR_NOMEM;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return LFMM_ERR_HIP;
+    }
+    c->stream = c->own_stream;
+    *out = c;
+    return LFMM_OK;
+}
+
+void lfmm_destroy(lfmm_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->arena.release();
+    for (auto& kv : c->tables) (void)hipFree(kv.second.block);
+    for (auto e : c->events) (void)hipEventDestroy(e);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+}
+
+const char* lfmm_strerror(int s)
+{
+    switch (s) {
+    case LFMM_OK: return "ok";
+    case LFMM_ERR_INVALID: return "invalid argument";
+    case LFMM_ERR_TERMS: return "expansion terms out of range";
+    case LFMM_ERR_DEPTH: return "tree depth out of range";
+    case LFMM_ERR_NOMEM: return "out of device memory";
+    case LFMM_ERR_HIP: return "HIP runtime error";
+    case LFMM_ERR_NONFINITE: return "non-finite input";
+    case LFMM_ERR_NODEVICE: return "no usable GPU";
+    case LFMM_ERR_UNSUPPORTED: return "unsupported option";
+    default: return "unknown status";
+    }
+}
+
+int lfmm_set_stream(lfmm_ctx* c, void* s)
+{
+    if (!c) return LFMM_ERR_INVALID;
+    c->stream = s ? (hipStream_t)s : c->own_stream;
+    return LFMM_OK;
+}
+
+int lfmm_trim(lfmm_ctx* c)
+{
+    if (!c) return LFMM_ERR_INVALID;
+    std::lock_guard<std::mutex> g(c->mu);
+    (void)hipSetDevice(c->device);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return LFMM_ERR_HIP;
+    c->arena.release();
+    return LFMM_OK;
+}
+
+int lfmm_fmm3d(lfmm_ctx* c, const lfmm_points3d* sources, const lfmm_points3d* targets, double eps, int terms,
+               int tree_depth, int leaf_max, int memory, unsigned flags, double* potential, double* field,
+               lfmm_report* report)
+{
+    if (!c || !valid_points(sources, true)) return LFMM_ERR_INVALID;
+    if (memory != LFMM_MEM_HOST && memory != LFMM_MEM_DEVICE) return LFMM_ERR_INVALID;
+    if (terms != LFMM_AUTO && terms < 1) return LFMM_ERR_TERMS;
+    if (tree_depth != LFMM_AUTO && (tree_depth < 1 || tree_depth > kMaxDepth)) return LFMM_ERR_DEPTH;
+    if (!(eps > 0.0) && terms == LFMM_AUTO) return LFMM_ERR_INVALID;
+    if (targets && targets->n > 0) return LFMM_ERR_UNSUPPORTED;
+    if (report) std::memset(report, 0, sizeof(*report));
+    if (sources->n == 0) return LFMM_OK;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return LFMM_ERR_HIP;
+    try {
+        return fmm3d_self(c, sources, eps, terms, tree_depth, leaf_max, memory, flags, potential, field, report);
+    } catch (const std::bad_alloc&) {
+        return LFMM_ERR_NOMEM;
+    } catch (...) {
+        return LFMM_ERR_INVALID;
+    }
+}
+
+int lfmm_tree3d(lfmm_ctx* c, const lfmm_points3d* particles, int tree_depth, int leaf_max, int32_t* depth_out,
+                int64_t* level_sizes, double* geom, int64_t* pos_map, uint64_t* keys, int64_t* begins)
+{
+    if (!c || !valid_points(particles, false) || particles->n == 0) return LFMM_ERR_INVALID;
+    if (tree_depth != LFMM_AUTO && (tree_depth < 1 || tree_depth > kMaxDepth)) return LFMM_ERR_DEPTH;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return LFMM_ERR_HIP;
+    try {
+        c->arena.reset();
+        Tree T;
+        lfmm_points3d P = *particles;
+        P.q = nullptr;  // the tree needs positions only
+        int rc = build_tree(c, &P, LFMM_MEM_HOST, tree_depth, leaf_max, T);
+        if (rc != LFMM_OK) return rc;
+        hipStream_t st = c->stream;
+        const int64_t n = T.n;
+        if (depth_out) *depth_out = T.depth;
+        if (level_sizes)
+            for (int l = 0; l < T.depth; ++l) level_sizes[l] = T.cnt[l];
+        if (geom) LFMM_HIP(hipMemcpyAsync(geom, T.geom, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (pos_map) {
+            std::vector<uint32_t> ord(n);
+            LFMM_HIP(hipMemcpyAsync(ord.data(), T.order, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            LFMM_HIP(hipStreamSynchronize(st));
+            for (int64_t s = 0; s < n; ++s) pos_map[ord[s]] = s;
+        }
+        int64_t ko = 0, bo = 0;
+        for (int l = 0; l < T.depth; ++l) {
+            if (keys) LFMM_HIP(hipMemcpyAsync(keys + ko, T.lkeys[l], T.cnt[l] * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+            if (begins) {
+                std::vector<int32_t> b(T.cnt[l] + 1);
+                LFMM_HIP(hipMemcpyAsync(b.data(), T.lbegin[l], (T.cnt[l] + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+                LFMM_HIP(hipStreamSynchronize(st));
+                for (int64_t i = 0; i <= T.cnt[l]; ++i) begins[bo + i] = b[i];
+            }
+            ko += T.cnt[l];
+            bo += T.cnt[l] + 1;
+        }
+        LFMM_HIP(hipStreamSynchronize(st));
+        return LFMM_OK;
+    } catch (const std::bad_alloc&) {
+        return LFMM_ERR_NOMEM;
+    } catch (...) {
+        return LFMM_ERR_INVALID;
+    }
+}
+
+int lfmm_host_wigner_d(int j, double beta, double* out)
+{
+    if (j < 0 || j > 64 || !out) return LFMM_ERR_INVALID;
+    host_wigner_small_d(j, beta, out);
+    return LFMM_OK;
+}
+
+int64_t lfmm_host_op_tables(int N, int which, void* out, int64_t capacity)
+{
+    if (N < 1 || N > LFMM_MAX_TERMS_3D) return -(int64_t)LFMM_ERR_TERMS;
+    try {
+        OpTables T;
+        build_op_tables(N, T);
+        const void* p = nullptr;
+        int64_t count = 0;
+        size_t esz = 8;
+        switch (which) {
+        case 0: p = T.rot.data(); count = (int64_t)T.rot.size(); break;
+        case 1: p = T.ephi.data(); count = (int64_t)T.ephi.size(); break;
+        case 2: p = T.trans.data(); count = (int64_t)T.trans.size(); break;
+        case 3: p = T.dir_theta.data(); count = (int64_t)T.dir_theta.size(); esz = 4; break;
+        case 4: p = T.dir_trans.data(); count = (int64_t)T.dir_trans.size(); esz = 4; break;
+        case 5: p = T.dir_vec.data(); count = (int64_t)T.dir_vec.size(); esz = 4; break;
+        case 6: p = T.class_dirs.data(); count = (int64_t)T.class_dirs.size(); esz = 4; break;
+        case 7: p = T.knm.data(); count = (int64_t)T.knm.size(); break;
+        default: return -(int64_t)LFMM_ERR_INVALID;
+        }
+        if (out && capacity >= count) std::memcpy(out, p, esz * (size_t)count);
+        return count;
+    } catch (...) {
+        return -(int64_t)LFMM_ERR_NOMEM;
+    }
+}
+
+const char* lfmm_version(void) { return "lfmm-mi355x 0.1.0 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// device_common.h -- device helpers shared by the tree and operator kernels
+// (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace lfmm {
+
+constexpr int kWave = 64;
+
+// ---- Morton interleave: x is the high bit of each triple, as in the
+// reference's MortonCode (src/cpu/morton_tree.cpp:93-99). 21 bits per axis.
+__host__ __device__ inline uint64_t spread3(uint64_t v)
+{
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+__host__ __device__ inline uint32_t compact3(uint64_t v)
+{
+    v &= 0x1249249249249249ull;
+    v = (v ^ (v >> 2)) & 0x10c30c30c30c30c3ull;
+    v = (v ^ (v >> 4)) & 0x100f00f00f00f00full;
+    v = (v ^ (v >> 8)) & 0x1f0000ff0000ffull;
+    v = (v ^ (v >> 16)) & 0x1f00000000ffffull;
+    v = (v ^ (v >> 32)) & 0x1fffffull;
+    return (uint32_t)v;
+}
+
+__host__ __device__ inline uint64_t morton3(uint32_t ix, uint32_t iy, uint32_t iz)
+{
+    return (spread3(ix) << 2) | (spread3(iy) << 1) | spread3(iz);
+}
+
+// Box centre exactly as the reference's PointFromMortonCode
+// (src/cpu/morton_tree.cpp:122-145): ((2i+1)/2^(l+1)) * scale + shift with
+// the product and the sum rounded separately.
+__device__ inline double box_centre(uint32_t i, int level, double scale, double shift)
+{
+    const double frac = (double)(2u * i + 1u) * __longlong_as_double((int64_t)(1023 - (level + 1)) << 52);
+    return __dadd_rn(__dmul_rn(frac, scale), shift);
+}
+
+// ---- DPP row (16-lane) reductions -------------------------------------
+template <int CTRL>
+__device__ inline double dpp_d(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int l2 = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xf, 0xf, false);
+    const int h2 = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xf, 0xf, false);
+    return __hiloint2double(h2, l2);
+}
+
+// Sum over the 16 lanes of a DPP row; every lane of the row gets the sum.
+__device__ inline double row16_sum(double v)
+{
+    v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_d<0x141>(v);  // row_half_mirror
+    v += dpp_d<0x140>(v);  // row_mirror
+    return v;
+}
+
+__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 cmulc(double2 a, double2 b)  // a * conj(b)
+{
+    return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+// Find box index by key at a level: dense table when present, otherwise a
+// binary search over the sorted unique keys.  Returns -1 when absent.
+__device__ inline int find_box(uint64_t key, const int* __restrict__ lut, const uint64_t* __restrict__ keys, int size)
+{
+    if (lut) return lut[key];
+    int lo = 0, hi = size;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const uint64_t k = keys[mid];
+        if (k < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < size && keys[lo] == key) ? lo : -1;
+}
+
+}  // namespace lfmm

@@ -1,0 +1,58 @@
+// lfmm_internal.h -- shared declarations of the MI355X Laplace FMM library.
+//
+// Expansion storage convention (3D): coefficient (n, m), 0 <= m <= n < N, at
+// index n(n+1)/2 + m as a double2 (re, im), boxes contiguous per level
+// ([box][coef]).  Coefficients are stored NORMALISED per level:
+//   multipole  Mt_nm = M_nm / h^n,   local  Lt_nm = L_nm * h^(n+1),
+// where h is the box edge at that level and M, L are the reference's
+// coefficients (src/cpu/multipole3d.cpp:20-49 defines M; the local expansion
+// is evaluated as sum (2-delta_m0) Re(L_nm r^n K_nm P_n^m e^{i m phi}),
+// :1152-1208).  With this scaling every translation operator depends only on
+// the integer box offset, never on the level.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace lfmm {
+
+constexpr int kMaxTerms3d = 18;
+constexpr int kNumM2LDirs = 316;   // offsets in [-3,3]^3 minus [-1,1]^3
+constexpr int kNumDiagDirs = 8;    // (+-1, +-1, +-1): child <-> parent
+constexpr int kNumDirs = kNumM2LDirs + kNumDiagDirs;
+constexpr int kDiagBase = kNumM2LDirs;
+constexpr int kDirsPerClass = 189; // M2L offsets reachable from one child class
+constexpr int kNumTransM2L = 16;   // index = distance class (|d|^2 classes)
+constexpr int kTransM2M = kNumTransM2L;
+constexpr int kTransL2L = kNumTransM2L + 1;
+constexpr int kNumTrans = kNumTransM2L + 2;
+
+constexpr int tri(int n) { return n * (n + 1) / 2; }
+// sum_{n' < n} (n'+1)^2: start of degree n in a per-degree (n+1)x(n+1) table
+constexpr int rot_base(int n) { return n * (n + 1) * (2 * n + 1) / 6; }
+constexpr int rot_size(int N) { return rot_base(N); }
+// translation table: per order k an (N-k)x(N-k) block, rows = output degree
+constexpr int tr_base(int N, int k) { return rot_base(N) - rot_base(N - k); }
+constexpr int tr_size(int N) { return rot_base(N); }
+
+// Host-built FP64 operator tables for one expansion size N.
+struct OpTables {
+    int N = 0;
+    int num_theta = 0;
+    std::vector<double> rot;       // [theta][rot_size][2]: (P, Q) rotation pair
+    std::vector<double> ephi;      // [dir][N][2]: (cos k phi, sin k phi)
+    std::vector<double> trans;     // [kNumTrans][tr_size]
+    std::vector<int> dir_theta;    // [kNumDirs]
+    std::vector<int> dir_trans;    // [kNumDirs] (M2L class; -1 for diagonals)
+    std::vector<int> dir_vec;      // [kNumDirs][4]: dx, dy, dz, 0
+    std::vector<int> class_dirs;   // [8][kDirsPerClass]
+    std::vector<double> knm;       // [tri(N + 1)]
+};
+
+void build_op_tables(int N, OpTables& T);
+
+// Small host helpers exposed for tests through the C ABI debug entry points.
+double host_knm(int n, int m);
+double host_anm(int n, int m);
+void host_wigner_small_d(int j, double beta, double* out); // (2j+1)^2, [mp][m]
+
+}  // namespace lfmm

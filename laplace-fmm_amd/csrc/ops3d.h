@@ -1,0 +1,64 @@
+// ops3d.h -- argument blocks and launchers of the 3D operator kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace lfmm {
+
+struct P2MArgs {
+    const double4* src;        // sorted (x, y, z, q)
+    const int32_t* leaf_begin; // [nleaf + 1]
+    const uint64_t* leaf_key;
+    int nleaf;
+    int level;
+    const double* geom;        // {shift x, y, z, scale} (device)
+    const double* knm;         // K_nm table, tri(N+1)
+    double2* outer;            // normalised multipoles of the leaves
+};
+
+struct RtrTables {
+    const double2* rot;        // [theta][rot_size(N)] (P, Q)
+    const double2* ephi;       // [dir][N]
+    const double* trans;       // [kNumTrans][tr_size(N)]
+    const int* dir_theta;
+    const int* dir_trans;
+    const int* dir_vec;        // [dir][4]
+    const int* class_dirs;     // [8][189]
+};
+
+struct RtrLevel {
+    int level;                 // M2L: target level; M2M: parent level
+    int num_parents;           // M2L: boxes at level-1; M2M: boxes at level
+    const int32_t* child_begin;   // child ranges of those parents
+    const uint64_t* child_keys;   // keys of the child level
+    const int32_t* lut;        // M2L source lookup at `level` (dense) or null
+    const uint64_t* keys;      // keys at `level` (binary search fallback)
+    int size;                  // boxes at `level`
+    const double2* src;        // M2L: outer[level]; M2M: outer[level+1]
+    const double2* parent_local;  // M2L: inner[level-1] (null: no L2L)
+    double2* dst;              // M2L: inner[level]; M2M: outer[level]
+};
+
+struct LeafArgs {
+    const double4* src;
+    const int32_t* leaf_of;    // leaf index of each sorted particle
+    const uint64_t* leaf_key;
+    const int32_t* leaf_begin;
+    int nleaf;
+    int level;
+    const double* geom;
+    const int32_t* lut;        // dense key -> leaf table or null
+    const double2* local;      // normalised locals of the leaves
+    const double* knm;
+    const uint32_t* perm;      // sorted -> input index
+    int64_t n;
+    int exclude_self;
+    double* pot;
+    double* field;
+};
+
+hipError_t launch_p2m(int N, const P2MArgs& a, hipStream_t st);
+hipError_t launch_rtr(int N, int kind, const RtrTables& T, const RtrLevel& L, hipStream_t st);
+hipError_t launch_leaf(int N, bool local, const LeafArgs& a, hipStream_t st);
+
+}  // namespace lfmm

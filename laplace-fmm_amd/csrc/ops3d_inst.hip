@@ -1,0 +1,13 @@
+// ops3d_inst.hip -- instantiates the operator kernels for one expansion size
+// (LFMM_N, set by the build), so the 18 sizes compile in parallel.
+#include "ops3d_kernels.h"
+
+#ifndef LFMM_N
+#error "build with -DLFMM_N=<terms>"
+#endif
+
+namespace lfmm {
+template hipError_t do_p2m<LFMM_N>(const P2MArgs&, hipStream_t);
+template hipError_t do_rtr<LFMM_N>(int, const RtrTables&, const RtrLevel&, hipStream_t);
+template hipError_t do_leaf<LFMM_N>(bool, const LeafArgs&, hipStream_t);
+}  // namespace lfmm

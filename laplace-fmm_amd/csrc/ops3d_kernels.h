@@ -1,0 +1,555 @@
+// ops3d_kernels.h -- FP64 FMM operator kernels for gfx950 (CDNA4, wave64).
+// Compiled once per expansion size by ops3d_inst.hip (-DLFMM_N=<terms>).
+//
+//   k_p2m   P2M, wave per leaf           (reference Multipole, src/cpu/multipole3d.cpp:20-49)
+//   k_rtr   M2M and M2L(+L2L)            (reference M2M :115-150, M2L :152-190, L2L :192-226)
+//   k_leaf  fused P2P + L2P, lane per target particle, results scattered to input order
+//           (reference ComputeLeaves<double> :855-1294 and the un-permute :1288-1293)
+//
+// Coefficients are normalised per level (lfmm_internal.h), so every operator
+// table depends only on an integer box offset.
+//
+// k_rtr is the rotate / axial-translate / rotate-back operator.  A workgroup
+// of 4 waves owns 64 target boxes (one per lane, the same lane in every
+// wave) and walks the interaction directions in lock step, so all rotation
+// and translation coefficients of a step are wave-uniform (scalar loads).
+// Per direction the op state (one expansion per lane) lives in LDS as
+// [coefficient][lane] and is produced / consumed in three phases that are
+// partitioned over the 4 waves in three different ways:
+//   A  forward rotation  Rz(phi), Ry(theta)        by source degree n
+//   B  axial translation                            by order m
+//   C  inverse rotation Ry(-theta), Rz(-phi), accumulate in registers  by target degree j
+// Each wave keeps only the accumulators of its own target degrees.
+#pragma once
+#include "device_common.h"
+#include "lfmm_internal.h"
+#include "ops3d.h"
+
+#include <type_traits>
+#include <utility>
+
+namespace lfmm {
+
+template <int B, int E, typename F>
+__device__ __forceinline__ void sfor(F&& f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        sfor<B + 1, E>(f);
+    }
+}
+
+// Work partition of degrees (phases A, C: cost (n+1)^2) and orders (phase B:
+// cost (N-m)^2) over NW waves, longest-processing-time first.
+template <int N, int NW>
+struct Partition {
+    int deg[kMaxTerms3d + 1];
+    int ord[kMaxTerms3d + 1];
+    constexpr Partition() : deg(), ord()
+    {
+        long la[NW] = {}, lb[NW] = {};
+        for (int n = N - 1; n >= 0; --n) {
+            int b = 0;
+            for (int w = 1; w < NW; ++w)
+                if (la[w] < la[b]) b = w;
+            deg[n] = b;
+            la[b] += (long)(n + 1) * (n + 1);
+        }
+        for (int m = 0; m < N; ++m) {
+            int b = 0;
+            for (int w = 1; w < NW; ++w)
+                if (lb[w] < lb[b]) b = w;
+            ord[m] = b;
+            lb[b] += (long)(N - m) * (N - m);
+        }
+    }
+};
+
+// ===================================================================== P2M
+// Mt_nm = sum_i q_i K_nm conj(S_n^m(u_i)), u = (x - centre)/h, where
+// S_n^m(u) = |u|^n P_n^m(cos theta) e^{i m phi} (Condon-Shortley P) is built
+// with the Cartesian recurrences
+//   S_m^m = -(2m-1)(ux + i uy) S_{m-1}^{m-1},  S_{m+1}^m = (2m+1) uz S_m^m,
+//   (n-m) S_n^m = (2n-1) uz S_{n-1}^m - (n+m-1) |u|^2 S_{n-2}^m.
+// A wave handles one leaf: 16 particle slots x 4 coefficient groups; the 16
+// partial sums of a group are reduced with DPP inside the 16-lane row.
+template <int N>
+__global__ void __launch_bounds__(256) k_p2m(P2MArgs a, const double* __restrict__ knm)
+{
+    constexpr int NX = tri(N);
+    constexpr int CH = (NX + 3) / 4;
+    const int lane = threadIdx.x & 63;
+    const int leaf = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (leaf >= a.nleaf) return;
+    const int slot = lane & 15, grp = lane >> 4;
+    const uint64_t key = a.leaf_key[leaf];
+    const double scale = a.geom[3];
+    const double cx = box_centre(compact3(key >> 2), a.level, scale, a.geom[0]);
+    const double cy = box_centre(compact3(key >> 1), a.level, scale, a.geom[1]);
+    const double cz = box_centre(compact3(key), a.level, scale, a.geom[2]);
+    const double ih = ldexp(1.0, a.level) / scale;
+    double2 acc[CH];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) acc[k] = make_double2(0.0, 0.0);
+    const int b = a.leaf_begin[leaf], e = a.leaf_begin[leaf + 1];
+    for (int p = b + slot; p < e; p += 16) {
+        const double4 s = a.src[p];
+        const double ux = (s.x - cx) * ih, uy = (s.y - cy) * ih, uz = (s.z - cz) * ih, q = s.w;
+        const double r2 = ux * ux + uy * uy + uz * uz;
+        auto put = [&](auto I, double2 S) {
+            constexpr int idx = decltype(I)::value;
+            constexpr int g = idx / CH, k = idx % CH;
+            const double f = q * knm[idx];
+            const double vr = (grp == g) ? f * S.x : 0.0;
+            const double vi = (grp == g) ? -f * S.y : 0.0;
+            acc[k].x += vr;
+            acc[k].y += vi;
+        };
+        double2 smm = make_double2(1.0, 0.0);
+        sfor<0, N>([&](auto M) {
+            constexpr int m = decltype(M)::value;
+            if constexpr (m > 0) {
+                const double c = -(double)(2 * m - 1);
+                smm = make_double2(c * (ux * smm.x - uy * smm.y), c * (ux * smm.y + uy * smm.x));
+            }
+            put(std::integral_constant<int, tri(m) + m>{}, smm);
+            if constexpr (m + 1 < N) {
+                double2 s2 = smm;
+                double2 s1 = make_double2((2 * m + 1) * uz * smm.x, (2 * m + 1) * uz * smm.y);
+                put(std::integral_constant<int, tri(m + 1) + m>{}, s1);
+                sfor<m + 2, N>([&](auto NN) {
+                    constexpr int n = decltype(NN)::value;
+                    constexpr double c1 = (double)(2 * n - 1) / (double)(n - m);
+                    constexpr double c2 = (double)(n + m - 1) / (double)(n - m);
+                    const double2 s0 = make_double2(c1 * uz * s1.x - c2 * r2 * s2.x, c1 * uz * s1.y - c2 * r2 * s2.y);
+                    put(std::integral_constant<int, tri(n) + m>{}, s0);
+                    s2 = s1;
+                    s1 = s0;
+                });
+            }
+        });
+    }
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+        acc[k].x = row16_sum(acc[k].x);
+        acc[k].y = row16_sum(acc[k].y);
+    }
+    double2* out = a.outer + (int64_t)leaf * NX;
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+        const int idx = grp * CH + k;
+        if (slot == (k & 15) && idx < NX) out[idx] = acc[k];
+    }
+}
+
+// ===================================================================== rotate-translate-rotate
+enum { KIND_M2M = 0, KIND_M2L = 1 };
+
+// Targets per workgroup: 64 (one per lane) while the FP64 op state of 64
+// targets fits the 160 KiB LDS, fewer lanes above that (N = 18).
+constexpr int rtr_tile(int N) { return tri(N) * 64 * 16 <= 160 * 1024 ? 64 : (160 * 1024) / (tri(N) * 16) / 8 * 8; }
+enum { TR_M2L = 0, TR_M2M = 1, TR_L2L = 2 };
+
+template <int N, int W, int TK>
+__device__ __forceinline__ void rtr_phase_b(const double* __restrict__ tt, double2* lds, int lane)
+{
+    constexpr Partition<N, 4> PT{};
+    constexpr int TW = rtr_tile(N);
+    if (lane >= TW) return;
+    sfor<0, N>([&](auto M) {
+        constexpr int m = decltype(M)::value;
+        if constexpr (PT.ord[m] == W) {
+            constexpr int L = N - m;
+            double2 x[L];
+            sfor<0, L>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                x[i] = lds[(tri(m + i) + m) * TW + lane];
+            });
+            const double* t = tt + tr_base(N, m);
+            sfor<0, L>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                double yr = 0.0, yi = 0.0;
+                sfor<0, L>([&](auto I) {
+                    constexpr int i = decltype(I)::value;
+                    constexpr bool nz = (TK == TR_M2L) || (TK == TR_M2M && i <= j) || (TK == TR_L2L && i >= j);
+                    if constexpr (nz) {
+                        const double c = t[j * L + i];
+                        yr = fma(c, x[i].x, yr);
+                        yi = fma(c, x[i].y, yi);
+                    }
+                });
+                lds[(tri(m + j) + m) * TW + lane] = make_double2(yr, yi);
+            });
+        }
+    });
+}
+
+// The operator tables arrive as separate __restrict__ kernel arguments so the
+// compiler can prove them read-only and fetch the wave-uniform coefficients
+// with scalar loads.
+#define RTR_TABLE_PARAMS                                                                             \
+    const double2* __restrict__ Trot, const double2* __restrict__ Tephi, const double* __restrict__ Ttrans, \
+        const int* __restrict__ Tdir_theta, const int* __restrict__ Tdir_trans, const int* __restrict__ Tdir_vec, \
+        const int* __restrict__ Tclass_dirs
+#define RTR_TABLE_ARGS Trot, Tephi, Ttrans, Tdir_theta, Tdir_trans, Tdir_vec, Tclass_dirs
+
+template <int N, int KIND, int W>
+__device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, double2* lds, int lane)
+{
+    constexpr int NX = tri(N);
+    constexpr int RS = rot_size(N);
+    constexpr int TS = tr_size(N);
+    constexpr Partition<N, 4> PT{};
+    constexpr int TW = rtr_tile(N);
+    const bool lok = lane < TW;
+    int tile, c;
+    if constexpr (KIND == KIND_M2L) {
+        tile = blockIdx.x >> 3;
+        c = blockIdx.x & 7;
+    } else {
+        tile = blockIdx.x;
+        c = 0;
+    }
+    const int P = tile * TW + lane;
+    const bool pvalid = lok && P < L.num_parents;
+    int cb = 0, ce = 0;
+    if (pvalid) {
+        cb = L.child_begin[P];
+        ce = L.child_begin[P + 1];
+    }
+    int tgt = -1;
+    uint32_t tx = 0, ty = 0, tz = 0;
+    if constexpr (KIND == KIND_M2L) {
+        for (int i = cb; i < ce; ++i)
+            if ((int)(L.child_keys[i] & 7u) == c) tgt = i;
+        if (tgt >= 0) {
+            const uint64_t k = L.child_keys[tgt];
+            tx = compact3(k >> 2);
+            ty = compact3(k >> 1);
+            tz = compact3(k);
+        }
+    } else {
+        tgt = pvalid ? P : -1;
+    }
+    const int side = 1 << L.level;
+    double2 acc[NX];
+#pragma unroll
+    for (int i = 0; i < NX; ++i) acc[i] = make_double2(0.0, 0.0);
+
+    const int has_l2l = (KIND == KIND_M2L && L.parent_local) ? 1 : 0;
+    const int ndir = (KIND == KIND_M2M) ? 8 : (kDirsPerClass + has_l2l);
+    for (int d = 0; d < ndir; ++d) {
+        int sidx = -1, dir, tk;
+        const double2* __restrict__ src;
+        if constexpr (KIND == KIND_M2M) {
+            dir = kDiagBase + d;
+            tk = TR_M2M;
+            src = L.src;
+            for (int i = cb; i < ce; ++i)
+                if ((int)(L.child_keys[i] & 7u) == d) sidx = i;
+        } else {
+            if (has_l2l && d == 0) {
+                dir = kDiagBase + (7 - c);
+                tk = TR_L2L;
+                src = L.parent_local;
+                sidx = tgt >= 0 ? P : -1;
+            } else {
+                dir = Tclass_dirs[c * kDirsPerClass + (d - has_l2l)];
+                tk = TR_M2L;
+                src = L.src;
+                if (tgt >= 0) {
+                    const int sx = (int)tx + Tdir_vec[4 * dir], sy = (int)ty + Tdir_vec[4 * dir + 1],
+                              sz = (int)tz + Tdir_vec[4 * dir + 2];
+                    if (sx >= 0 && sy >= 0 && sz >= 0 && sx < side && sy < side && sz < side)
+                        sidx = find_box(morton3(sx, sy, sz), L.lut, L.keys, L.size);
+                }
+            }
+        }
+        if (__ballot(sidx >= 0) == 0ull) continue;  // same lanes in every wave: uniform
+        const double2* __restrict__ rot = Trot + (int64_t)Tdir_theta[dir] * RS;
+        const double2* __restrict__ eph = Tephi + dir * N;
+        const int tsel = (tk == TR_M2L) ? Tdir_trans[dir] : (tk == TR_M2M ? kTransM2M : kTransL2L);
+        const double* __restrict__ tt = Ttrans + (int64_t)tsel * TS;
+        const double2* sp = src + (int64_t)(sidx < 0 ? 0 : sidx) * NX;
+
+        // ---- phase A: Rz(phi) then Ry(theta), by source degree
+        sfor<0, N>([&](auto NN) {
+            constexpr int n = decltype(NN)::value;
+            if constexpr (PT.deg[n] == W) {
+                double2 av[n + 1];
+                sfor<0, n + 1>([&](auto K) {
+                    constexpr int k = decltype(K)::value;
+                    double2 v = sidx >= 0 ? sp[tri(n) + k] : make_double2(0.0, 0.0);
+                    if constexpr (k > 0) v = cmul(v, eph[k]);
+                    av[k] = v;
+                });
+                const double2* r = rot + rot_base(n);
+                sfor<0, n + 1>([&](auto M) {
+                    constexpr int m = decltype(M)::value;
+                    double re = 0.0, im = 0.0;
+                    sfor<0, n + 1>([&](auto K) {
+                        constexpr int k = decltype(K)::value;
+                        const double2 pq = r[m * (n + 1) + k];
+                        re = fma(pq.x, av[k].x, re);
+                        im = fma(pq.y, av[k].y, im);
+                    });
+                    if (lok) lds[(tri(n) + m) * TW + lane] = make_double2(re, im);
+                });
+            }
+        });
+        __syncthreads();
+        // ---- phase B: axial translation, by order
+        if constexpr (KIND == KIND_M2M) {
+            rtr_phase_b<N, W, TR_M2M>(tt, lds, lane);
+        } else {
+            if (tk == TR_L2L) rtr_phase_b<N, W, TR_L2L>(tt, lds, lane);
+            else rtr_phase_b<N, W, TR_M2L>(tt, lds, lane);
+        }
+        __syncthreads();
+        // ---- phase C: Ry(-theta) then Rz(-phi), accumulate, by target degree
+        sfor<0, N>([&](auto JJ) {
+            constexpr int j = decltype(JJ)::value;
+            if constexpr (PT.deg[j] == W) {
+                double2 yv[j + 1];
+                sfor<0, j + 1>([&](auto M) {
+                    constexpr int m = decltype(M)::value;
+                    yv[m] = lok ? lds[(tri(j) + m) * TW + lane] : make_double2(0.0, 0.0);
+                });
+                const double2* r = rot + rot_base(j);
+                sfor<0, j + 1>([&](auto MU) {
+                    constexpr int mu = decltype(MU)::value;
+                    double re = 0.0, im = 0.0;
+                    sfor<0, j + 1>([&](auto M) {
+                        constexpr int m = decltype(M)::value;
+                        const double2 pq = r[mu * (j + 1) + m];
+                        if constexpr (((mu - m) & 1) == 0) {
+                            re = fma(pq.x, yv[m].x, re);
+                            im = fma(pq.y, yv[m].y, im);
+                        } else {
+                            re = fma(-pq.x, yv[m].x, re);
+                            im = fma(-pq.y, yv[m].y, im);
+                        }
+                    });
+                    double2 z = make_double2(re, im);
+                    if constexpr (mu > 0) z = cmulc(z, eph[mu]);
+                    acc[tri(j) + mu].x += z.x;
+                    acc[tri(j) + mu].y += z.y;
+                });
+            }
+        });
+        __syncthreads();
+    }
+    if (tgt >= 0) {
+        double2* out = L.dst + (int64_t)tgt * NX;
+        sfor<0, N>([&](auto JJ) {
+            constexpr int j = decltype(JJ)::value;
+            if constexpr (PT.deg[j] == W) {
+                sfor<0, j + 1>([&](auto MU) {
+                    constexpr int mu = decltype(MU)::value;
+                    out[tri(j) + mu] = acc[tri(j) + mu];
+                });
+            }
+        });
+    }
+}
+
+template <int N, int KIND>
+__global__ void __launch_bounds__(256) k_rtr(RTR_TABLE_PARAMS, RtrLevel L)
+{
+    __shared__ double2 lds[tri(N) * rtr_tile(N)];
+    const int lane = threadIdx.x & 63;
+    switch (threadIdx.x >> 6) {
+    case 0: rtr_wave<N, KIND, 0>(RTR_TABLE_ARGS, L, lds, lane); break;
+    case 1: rtr_wave<N, KIND, 1>(RTR_TABLE_ARGS, L, lds, lane); break;
+    case 2: rtr_wave<N, KIND, 2>(RTR_TABLE_ARGS, L, lds, lane); break;
+    default: rtr_wave<N, KIND, 3>(RTR_TABLE_ARGS, L, lds, lane); break;
+    }
+}
+
+// ===================================================================== P2P + L2P
+// Pair term of the reference (src/common/simple_math.h:130-139, numerics of
+// avx.h:128-146): q / max(r, 1e-4) and q (x - y) / max(r, 1e-4)^3.  1/r is
+// v_rsq_f64 refined by two Newton steps (full FP64 accuracy).
+__device__ __forceinline__ double rsqrt_fp64(double r2)
+{
+    double y = __builtin_amdgcn_rsq(r2);
+    const double hr = 0.5 * r2;
+    y = y * fma(-hr * y, y, 1.5);
+    y = y * fma(-hr * y, y, 1.5);
+    return y;
+}
+
+// L2P with normalised locals: phi = (1/h) Re G(u), G = sum w_m Lt_nm K_nm S_n^m(u),
+// field = -(1/h^2) grad_u Re G, using
+//   d/dz S_n^m = (n+m) S_{n-1}^m,  (d/dx + i d/dy) S_n^m = S_{n-1}^{m+1},
+//   (d/dx - i d/dy) S_n^m = -(n+m)(n+m-1) S_{n-1}^{m-1}  (m >= 1),
+//   (d/dx - i d/dy) S_n^0 = conj(S_{n-1}^1).
+// The field is the Cartesian gradient, so it stays finite on the box axis
+// where the reference's spherical form divides by sin(theta)
+// (src/cpu/multipole3d.cpp:1205-1206).
+template <int N>
+__device__ __forceinline__ void l2p_eval(const double2* __restrict__ Lt, const double* __restrict__ knm, double ux, double uy,
+                                         double uz, double& phi, double& gx, double& gy, double& gz)
+{
+    const double r2 = ux * ux + uy * uy + uz * uz;
+    double ph = 0.0;
+    double2 dz = make_double2(0.0, 0.0), dp = make_double2(0.0, 0.0), dm = make_double2(0.0, 0.0);
+    auto coef = [&](auto NI, auto MI) -> double2 {
+        constexpr int n = decltype(NI)::value, m = decltype(MI)::value;
+        const double2 l = Lt[tri(n) + m];
+        const double k = knm[tri(n) + m] * (m == 0 ? 1.0 : 2.0);
+        return make_double2(l.x * k, l.y * k);
+    };
+    auto use = [&](auto PI, auto QI, double2 S) {
+        constexpr int p = decltype(PI)::value, q = decltype(QI)::value;
+        {
+            const double2 c = coef(PI, QI);
+            ph += c.x * S.x - c.y * S.y;
+        }
+        if constexpr (p + 1 < N) {
+            const double2 cz = coef(std::integral_constant<int, p + 1>{}, QI);
+            const double2 t = cmul(cz, S);
+            dz.x += (double)(p + 1 + q) * t.x;
+            dz.y += (double)(p + 1 + q) * t.y;
+            if constexpr (q >= 1) {
+                const double2 cp = coef(std::integral_constant<int, p + 1>{}, std::integral_constant<int, q - 1>{});
+                const double2 u = cmul(cp, S);
+                dp.x += u.x;
+                dp.y += u.y;
+            }
+            {
+                const double2 cm = coef(std::integral_constant<int, p + 1>{}, std::integral_constant<int, q + 1>{});
+                const double2 u = cmul(cm, S);
+                constexpr double f = -(double)((p + q + 2) * (p + q + 1));
+                dm.x += f * u.x;
+                dm.y += f * u.y;
+            }
+            if constexpr (q == 1) {
+                const double2 c0 = coef(std::integral_constant<int, p + 1>{}, std::integral_constant<int, 0>{});
+                const double2 u = cmul(c0, make_double2(S.x, -S.y));
+                dm.x += u.x;
+                dm.y += u.y;
+            }
+        }
+    };
+    double2 smm = make_double2(1.0, 0.0);
+    sfor<0, N>([&](auto M) {
+        constexpr int m = decltype(M)::value;
+        // compiler memory fence: keeps each column's coefficient loads next to
+        // their use (otherwise they are merged and hoisted into one live set)
+        asm volatile("" ::: "memory");
+        if constexpr (m > 0) {
+            const double c = -(double)(2 * m - 1);
+            smm = make_double2(c * (ux * smm.x - uy * smm.y), c * (ux * smm.y + uy * smm.x));
+        }
+        use(std::integral_constant<int, m>{}, M, smm);
+        if constexpr (m + 1 < N) {
+            double2 s2 = smm;
+            double2 s1 = make_double2((2 * m + 1) * uz * smm.x, (2 * m + 1) * uz * smm.y);
+            use(std::integral_constant<int, m + 1>{}, M, s1);
+            sfor<m + 2, N>([&](auto NN) {
+                constexpr int n = decltype(NN)::value;
+                constexpr double c1 = (double)(2 * n - 1) / (double)(n - m);
+                constexpr double c2 = (double)(n + m - 1) / (double)(n - m);
+                const double2 s0 = make_double2(c1 * uz * s1.x - c2 * r2 * s2.x, c1 * uz * s1.y - c2 * r2 * s2.y);
+                use(NN, M, s0);
+                s2 = s1;
+                s1 = s0;
+            });
+        }
+    });
+    phi = ph;
+    gx = 0.5 * (dp.x + dm.x);
+    gy = 0.5 * (dp.y - dm.y);
+    gz = dz.x;
+}
+
+template <int N, bool LOCAL>
+__global__ void __launch_bounds__(256) k_leaf(LeafArgs a, const double2* __restrict__ local, const double* __restrict__ knm)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= a.n) return;
+    const int leaf = a.leaf_of[s];
+    const uint64_t key = a.leaf_key[leaf];
+    const uint32_t ix = compact3(key >> 2), iy = compact3(key >> 1), iz = compact3(key);
+    const double4 t = a.src[s];
+    const int side = 1 << a.level;
+    // far field first: its registers are dead before the near-field loop starts
+    double ph = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    if constexpr (LOCAL) {
+        const double scale = a.geom[3];
+        const double cx = box_centre(ix, a.level, scale, a.geom[0]);
+        const double cy = box_centre(iy, a.level, scale, a.geom[1]);
+        const double cz = box_centre(iz, a.level, scale, a.geom[2]);
+        const double ih = ldexp(1.0, a.level) / scale;
+        double lp, gx, gy, gz;
+        l2p_eval<N>(local + (int64_t)leaf * tri(N), knm, (t.x - cx) * ih, (t.y - cy) * ih, (t.z - cz) * ih, lp, gx, gy, gz);
+        ph = lp * ih;
+        const double ih2 = ih * ih;
+        fx = -gx * ih2;
+        fy = -gy * ih2;
+        fz = -gz * ih2;
+    }
+    for (int o = 0; o < 27; ++o) {
+        const int nx = (int)ix + o / 9 - 1, ny = (int)iy + (o / 3) % 3 - 1, nz = (int)iz + o % 3 - 1;
+        if (nx < 0 || ny < 0 || nz < 0 || nx >= side || ny >= side || nz >= side) continue;
+        const int nb = (o == 13) ? leaf : find_box(morton3(nx, ny, nz), a.lut, a.leaf_key, a.nleaf);
+        if (nb < 0) continue;
+        const int b = a.leaf_begin[nb], e = a.leaf_begin[nb + 1];
+        for (int j = b; j < e; ++j) {
+            const double4 p = a.src[j];
+            if (a.exclude_self && j == (int)s) continue;
+            const double dx = t.x - p.x, dy = t.y - p.y, dzz = t.z - p.z;
+            const double r2 = fmax(fma(dx, dx, fma(dy, dy, dzz * dzz)), 1.e-8);
+            const double ir = rsqrt_fp64(r2);
+            const double qi = p.w * ir;
+            const double qi3 = qi * (ir * ir);
+            ph += qi;
+            fx = fma(dx, qi3, fx);
+            fy = fma(dy, qi3, fy);
+            fz = fma(dzz, qi3, fz);
+        }
+    }
+    const int64_t o = a.perm[s];
+    if (a.pot) a.pot[o] = ph;
+    if (a.field) {
+        a.field[3 * o] = fx;
+        a.field[3 * o + 1] = fy;
+        a.field[3 * o + 2] = fz;
+    }
+}
+
+// ===================================================================== launchers
+template <int N>
+hipError_t do_p2m(const P2MArgs& a, hipStream_t st)
+{
+    const int nb = (a.nleaf + 3) / 4;
+    if (nb > 0) hipLaunchKernelGGL(k_p2m<N>, dim3(nb), dim3(256), 0, st, a, a.knm);
+    return hipGetLastError();
+}
+
+template <int N>
+hipError_t do_rtr(int kind, const RtrTables& T, const RtrLevel& L, hipStream_t st)
+{
+    const int tiles = (L.num_parents + rtr_tile(N) - 1) / rtr_tile(N);
+    if (tiles == 0) return hipSuccess;
+    if (kind == KIND_M2M)
+        hipLaunchKernelGGL((k_rtr<N, KIND_M2M>), dim3(tiles), dim3(256), 0, st, T.rot, T.ephi, T.trans, T.dir_theta,
+                           T.dir_trans, T.dir_vec, T.class_dirs, L);
+    else
+        hipLaunchKernelGGL((k_rtr<N, KIND_M2L>), dim3(tiles * 8), dim3(256), 0, st, T.rot, T.ephi, T.trans, T.dir_theta,
+                           T.dir_trans, T.dir_vec, T.class_dirs, L);
+    return hipGetLastError();
+}
+
+template <int N>
+hipError_t do_leaf(bool local, const LeafArgs& a, hipStream_t st)
+{
+    const int nb = (int)((a.n + 255) / 256);
+    if (nb == 0) return hipSuccess;
+    if (local) hipLaunchKernelGGL((k_leaf<N, true>), dim3(nb), dim3(256), 0, st, a, a.local, a.knm);
+    else hipLaunchKernelGGL((k_leaf<N, false>), dim3(nb), dim3(256), 0, st, a, a.local, a.knm);
+    return hipGetLastError();
+}
+
+}  // namespace lfmm
