@@ -67,11 +67,23 @@ def build(jobs=8, force=False):
     subprocess.run(cmd, check=True)
 
 
+def _share_torch_hip_runtime():
+    """torch's ROCm build ships its own libamdhip64 (loaded under the name
+    libamdhip64.so); liblfmm.so asks for libamdhip64.so.7.  Loading torch first
+    makes the dynamic loader bind liblfmm.so to that same runtime, so the
+    process holds one HIP runtime and torch tensors and lfmm share devices."""
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise FileNotFoundError("liblfmm.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        _share_torch_hip_runtime()
         L = ctypes.CDLL(LIB_PATH)
         L.lfmm_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
         L.lfmm_create.restype = ctypes.c_int
