@@ -193,6 +193,42 @@ __device__ __forceinline__ void rtr_phase_b(const double* __restrict__ tt, doubl
         const int* __restrict__ Tclass_dirs
 #define RTR_TABLE_ARGS Trot, Tephi, Ttrans, Tdir_theta, Tdir_trans, Tdir_vec, Tclass_dirs
 
+// Blocks are dealt to the 8 XCDs round-robin; renumber them so that each XCD
+// walks a contiguous range of (parent tile, child class) work items, i.e. a
+// compact region of the octree whose source expansions stay in its L2.
+__device__ __forceinline__ int xcd_local_block(int b, int nb)
+{
+    const int per = nb >> 3, rem = nb & 7;
+    const int x = b & 7, slot = b >> 3;
+    return (x < rem) ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
+}
+
+// Barrier for LDS hand-over between the phases: waits for this wave's LDS
+// traffic only, so global prefetch loads stay in flight across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Offsets of a wave's phase-A source degrees in its prefetch buffer.
+template <int N, int W>
+struct PrefetchLayout {
+    int off[kMaxTerms3d + 1];
+    int count;
+    constexpr PrefetchLayout() : off(), count(0)
+    {
+        constexpr Partition<N, 4> PT{};
+        for (int n = 0; n < N; ++n) {
+            off[n] = count;
+            if (PT.deg[n] == W) count += n + 1;
+        }
+    }
+};
+
+struct RtrStep {
+    int sidx;                 // source box for this lane (-1: none)
+    int dir;                  // direction table index
+    int tk;                   // translation kind
+    const double2* src;       // source expansion array
+};
+
 template <int N, int KIND, int W>
 __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, double2* lds, int lane)
 {
@@ -200,14 +236,16 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
     constexpr int RS = rot_size(N);
     constexpr int TS = tr_size(N);
     constexpr Partition<N, 4> PT{};
+    constexpr PrefetchLayout<N, W> PF{};
     constexpr int TW = rtr_tile(N);
     const bool lok = lane < TW;
+    const int blk = xcd_local_block(blockIdx.x, gridDim.x);
     int tile, c;
     if constexpr (KIND == KIND_M2L) {
-        tile = blockIdx.x >> 3;
-        c = blockIdx.x & 7;
+        tile = blk >> 3;
+        c = blk & 7;
     } else {
-        tile = blockIdx.x;
+        tile = blk;
         c = 0;
     }
     const int P = tile * TW + lane;
@@ -238,106 +276,135 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
 
     const int has_l2l = (KIND == KIND_M2L && L.parent_local) ? 1 : 0;
     const int ndir = (KIND == KIND_M2M) ? 8 : (kDirsPerClass + has_l2l);
-    for (int d = 0; d < ndir; ++d) {
-        int sidx = -1, dir, tk;
-        const double2* __restrict__ src;
+
+    auto step_of = [&](int d) {
+        RtrStep s;
+        s.sidx = -1;
         if constexpr (KIND == KIND_M2M) {
-            dir = kDiagBase + d;
-            tk = TR_M2M;
-            src = L.src;
+            s.dir = kDiagBase + d;
+            s.tk = TR_M2M;
+            s.src = L.src;
             for (int i = cb; i < ce; ++i)
-                if ((int)(L.child_keys[i] & 7u) == d) sidx = i;
+                if ((int)(L.child_keys[i] & 7u) == d) s.sidx = i;
         } else {
             if (has_l2l && d == 0) {
-                dir = kDiagBase + (7 - c);
-                tk = TR_L2L;
-                src = L.parent_local;
-                sidx = tgt >= 0 ? P : -1;
+                s.dir = kDiagBase + (7 - c);
+                s.tk = TR_L2L;
+                s.src = L.parent_local;
+                s.sidx = tgt >= 0 ? P : -1;
             } else {
-                dir = Tclass_dirs[c * kDirsPerClass + (d - has_l2l)];
-                tk = TR_M2L;
-                src = L.src;
+                s.dir = Tclass_dirs[c * kDirsPerClass + (d - has_l2l)];
+                s.tk = TR_M2L;
+                s.src = L.src;
                 if (tgt >= 0) {
-                    const int sx = (int)tx + Tdir_vec[4 * dir], sy = (int)ty + Tdir_vec[4 * dir + 1],
-                              sz = (int)tz + Tdir_vec[4 * dir + 2];
+                    const int sx = (int)tx + Tdir_vec[4 * s.dir], sy = (int)ty + Tdir_vec[4 * s.dir + 1],
+                              sz = (int)tz + Tdir_vec[4 * s.dir + 2];
                     if (sx >= 0 && sy >= 0 && sz >= 0 && sx < side && sy < side && sz < side)
-                        sidx = find_box(morton3(sx, sy, sz), L.lut, L.keys, L.size);
+                        s.sidx = find_box(morton3(sx, sy, sz), L.lut, L.keys, L.size);
                 }
             }
         }
-        if (__ballot(sidx >= 0) == 0ull) continue;  // same lanes in every wave: uniform
-        const double2* __restrict__ rot = Trot + (int64_t)Tdir_theta[dir] * RS;
-        const double2* __restrict__ eph = Tephi + dir * N;
-        const int tsel = (tk == TR_M2L) ? Tdir_trans[dir] : (tk == TR_M2M ? kTransM2M : kTransL2L);
-        const double* __restrict__ tt = Ttrans + (int64_t)tsel * TS;
-        const double2* sp = src + (int64_t)(sidx < 0 ? 0 : sidx) * NX;
-
-        // ---- phase A: Rz(phi) then Ry(theta), by source degree
+        return s;
+    };
+    // the phase-A source coefficients of the next step, loaded one step ahead
+    double2 pf[PF.count > 0 ? PF.count : 1];
+    auto prefetch = [&](const RtrStep& s) {
+        const double2* sp = s.src + (int64_t)(s.sidx < 0 ? 0 : s.sidx) * NX;
         sfor<0, N>([&](auto NN) {
             constexpr int n = decltype(NN)::value;
             if constexpr (PT.deg[n] == W) {
-                double2 av[n + 1];
                 sfor<0, n + 1>([&](auto K) {
                     constexpr int k = decltype(K)::value;
-                    double2 v = sidx >= 0 ? sp[tri(n) + k] : make_double2(0.0, 0.0);
-                    if constexpr (k > 0) v = cmul(v, eph[k]);
-                    av[k] = v;
+                    pf[PF.off[n] + k] = s.sidx >= 0 ? sp[tri(n) + k] : make_double2(0.0, 0.0);
                 });
-                const double2* r = rot + rot_base(n);
-                sfor<0, n + 1>([&](auto M) {
-                    constexpr int m = decltype(M)::value;
-                    double re = 0.0, im = 0.0;
+            }
+        });
+    };
+
+    RtrStep nxt = step_of(0);
+    prefetch(nxt);
+    for (int d = 0; d < ndir; ++d) {
+        const RtrStep cur = nxt;
+        const bool any = __ballot(cur.sidx >= 0) != 0ull;  // same lanes in every wave: uniform
+        const double2* __restrict__ rot = Trot + (int64_t)Tdir_theta[cur.dir] * RS;
+        const double2* __restrict__ eph = Tephi + cur.dir * N;
+        if (any) {
+            // ---- phase A: Rz(phi) then Ry(theta), by source degree
+            sfor<0, N>([&](auto NN) {
+                constexpr int n = decltype(NN)::value;
+                if constexpr (PT.deg[n] == W) {
+                    double2 av[n + 1];
                     sfor<0, n + 1>([&](auto K) {
                         constexpr int k = decltype(K)::value;
-                        const double2 pq = r[m * (n + 1) + k];
-                        re = fma(pq.x, av[k].x, re);
-                        im = fma(pq.y, av[k].y, im);
+                        double2 v = pf[PF.off[n] + k];
+                        if constexpr (k > 0) v = cmul(v, eph[k]);
+                        av[k] = v;
                     });
-                    if (lok) lds[(tri(n) + m) * TW + lane] = make_double2(re, im);
-                });
+                    const double2* r = rot + rot_base(n);
+                    sfor<0, n + 1>([&](auto M) {
+                        constexpr int m = decltype(M)::value;
+                        double re = 0.0, im = 0.0;
+                        sfor<0, n + 1>([&](auto K) {
+                            constexpr int k = decltype(K)::value;
+                            const double2 pq = r[m * (n + 1) + k];
+                            re = fma(pq.x, av[k].x, re);
+                            im = fma(pq.y, av[k].y, im);
+                        });
+                        if (lok) lds[(tri(n) + m) * TW + lane] = make_double2(re, im);
+                    });
+                }
+            });
+            lds_barrier();
+            // ---- phase B: axial translation, by order
+            const int tsel = (cur.tk == TR_M2L) ? Tdir_trans[cur.dir] : (cur.tk == TR_M2M ? kTransM2M : kTransL2L);
+            const double* __restrict__ tt = Ttrans + (int64_t)tsel * TS;
+            if constexpr (KIND == KIND_M2M) {
+                rtr_phase_b<N, W, TR_M2M>(tt, lds, lane);
+            } else {
+                if (cur.tk == TR_L2L) rtr_phase_b<N, W, TR_L2L>(tt, lds, lane);
+                else rtr_phase_b<N, W, TR_M2L>(tt, lds, lane);
             }
-        });
-        __syncthreads();
-        // ---- phase B: axial translation, by order
-        if constexpr (KIND == KIND_M2M) {
-            rtr_phase_b<N, W, TR_M2M>(tt, lds, lane);
-        } else {
-            if (tk == TR_L2L) rtr_phase_b<N, W, TR_L2L>(tt, lds, lane);
-            else rtr_phase_b<N, W, TR_M2L>(tt, lds, lane);
+            lds_barrier();
         }
-        __syncthreads();
-        // ---- phase C: Ry(-theta) then Rz(-phi), accumulate, by target degree
-        sfor<0, N>([&](auto JJ) {
-            constexpr int j = decltype(JJ)::value;
-            if constexpr (PT.deg[j] == W) {
-                double2 yv[j + 1];
-                sfor<0, j + 1>([&](auto M) {
-                    constexpr int m = decltype(M)::value;
-                    yv[m] = lok ? lds[(tri(j) + m) * TW + lane] : make_double2(0.0, 0.0);
-                });
-                const double2* r = rot + rot_base(j);
-                sfor<0, j + 1>([&](auto MU) {
-                    constexpr int mu = decltype(MU)::value;
-                    double re = 0.0, im = 0.0;
+        // issue the next step's source loads; they complete under phase C
+        if (d + 1 < ndir) {
+            nxt = step_of(d + 1);
+            prefetch(nxt);
+        }
+        if (any) {
+            // ---- phase C: Ry(-theta) then Rz(-phi), accumulate, by target degree
+            sfor<0, N>([&](auto JJ) {
+                constexpr int j = decltype(JJ)::value;
+                if constexpr (PT.deg[j] == W) {
+                    double2 yv[j + 1];
                     sfor<0, j + 1>([&](auto M) {
                         constexpr int m = decltype(M)::value;
-                        const double2 pq = r[mu * (j + 1) + m];
-                        if constexpr (((mu - m) & 1) == 0) {
-                            re = fma(pq.x, yv[m].x, re);
-                            im = fma(pq.y, yv[m].y, im);
-                        } else {
-                            re = fma(-pq.x, yv[m].x, re);
-                            im = fma(-pq.y, yv[m].y, im);
-                        }
+                        yv[m] = lok ? lds[(tri(j) + m) * TW + lane] : make_double2(0.0, 0.0);
                     });
-                    double2 z = make_double2(re, im);
-                    if constexpr (mu > 0) z = cmulc(z, eph[mu]);
-                    acc[tri(j) + mu].x += z.x;
-                    acc[tri(j) + mu].y += z.y;
-                });
-            }
-        });
-        __syncthreads();
+                    const double2* r = rot + rot_base(j);
+                    sfor<0, j + 1>([&](auto MU) {
+                        constexpr int mu = decltype(MU)::value;
+                        double re = 0.0, im = 0.0;
+                        sfor<0, j + 1>([&](auto M) {
+                            constexpr int m = decltype(M)::value;
+                            const double2 pq = r[mu * (j + 1) + m];
+                            if constexpr (((mu - m) & 1) == 0) {
+                                re = fma(pq.x, yv[m].x, re);
+                                im = fma(pq.y, yv[m].y, im);
+                            } else {
+                                re = fma(-pq.x, yv[m].x, re);
+                                im = fma(-pq.y, yv[m].y, im);
+                            }
+                        });
+                        double2 z = make_double2(re, im);
+                        if constexpr (mu > 0) z = cmulc(z, eph[mu]);
+                        acc[tri(j) + mu].x += z.x;
+                        acc[tri(j) + mu].y += z.y;
+                    });
+                }
+            });
+            lds_barrier();
+        }
     }
     if (tgt >= 0) {
         double2* out = L.dst + (int64_t)tgt * NX;
