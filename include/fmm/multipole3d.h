@@ -13,7 +13,8 @@
 //     library builds its own tables on first use.
 // The computation runs inside the constructor, as in the reference.  Where
 // the reference has no error path (unchecked CUDA calls), this header throws
-// std::runtime_error carrying the library status.
+// std::runtime_error carrying the library status.  The (sources, targets)
+// constructor ignores the targets' charges (the reference requires q = 0).
 #pragma once
 
 #include <array>

@@ -114,8 +114,12 @@ int lfmm_trim(lfmm_ctx* ctx);
  *   potential: n_out doubles; field: n_out*3 doubles (x,y,z per particle);
  *     either may be NULL.  n_out = targets ? targets->n : sources->n.
  *   report: optional.
- * Single-set solves are implemented; target sets return
- * LFMM_ERR_UNSUPPORTED in this version.
+ * With targets (charges ignored), the solve follows the reference's
+ * source/target constructor: targets are treated as zero-charge particles of
+ * one tree (src/cpu/multipole3d.cpp:1311-1319) and get outputs; sources get
+ * none.  In a single-set solve every particle gets an output, also when some
+ * charges are exactly zero (the reference then evaluates only the q = 0
+ * particles, src/cpu/multipole3d.cpp:878; deliberate deviation).
  */
 int lfmm_fmm3d(lfmm_ctx* ctx, const lfmm_points3d* sources, const lfmm_points3d* targets,
                double eps, int terms, int tree_depth, int leaf_max,
@@ -126,8 +130,9 @@ int lfmm_fmm3d(lfmm_ctx* ctx, const lfmm_points3d* sources, const lfmm_points3d*
  * The Morton octree of a particle set, as built inside lfmm_fmm3d.  Replaces
  * the reference's MortonTree<3, point3d, double> (src/cpu/morton_tree.h:24-45,
  * GPU twin src/cuda/morton_tree.h): positions_map, level_sizes and the cell
- * keys / ranges of every level.  Inputs are host pointers (charges unused);
- * outputs are host arrays, any of which may be NULL:
+ * keys / ranges of every level.  Inputs are host pointers; with charges,
+ * q = 0 particles follow the others inside each leaf (morton_tree.cpp:426-468).
+ * Outputs are host arrays, any of which may be NULL:
  *   depth_out      number of levels (leaves at depth-1)
  *   level_sizes    [32] boxes per level
  *   geom           [4] {shift_x, shift_y, shift_z, scale} (morton_tree.cpp:293-301)

@@ -156,12 +156,13 @@ def _is_torch_cuda(x):
 
 
 def fmm3d(xyz, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exclude_self=False,
-          timings=False, count_ops=False, ctx=None, out=None):
-    """Single-set 3D FMM: potential (n,) and field (n,3) at every source.
+          timings=False, count_ops=False, ctx=None, out=None, targets=None):
+    """3D FMM: potential (m,) and field (m,3) at every target.
 
-    xyz (n,3) and q (n,) are float64 numpy arrays (host path) or CUDA torch
-    tensors (device path; outputs are then CUDA tensors, or written into
-    `out=(pot, field)`).  Returns (pot, field, report_dict).
+    xyz (n,3) and q (n,) are the sources; targets (m,3) defaults to the
+    sources themselves (m = n).  float64 numpy arrays take the host path,
+    CUDA torch tensors the device path (outputs are then CUDA tensors, or
+    written into `out=(pot, field)`).  Returns (pot, field, report_dict).
     """
     ctx = ctx or default_context()
     flags = ((FLAG_EXCLUDE_SELF if exclude_self else 0) | (FLAG_TIMINGS if timings else 0) |
@@ -175,12 +176,19 @@ def fmm3d(xyz, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exc
             raise ValueError("positions must be contiguous in the last dimension")
         n = xyz.shape[0]
         pts = Points3d(n, xyz.data_ptr(), xyz.stride(0), q.data_ptr(), q.stride(0))
+        tg = None
+        m = n
+        if targets is not None:
+            if targets.dtype != torch.float64 or targets.stride(-1) != 1:
+                raise TypeError("device targets must be float64, contiguous per point")
+            m = targets.shape[0]
+            tg = ctypes.byref(Points3d(m, targets.data_ptr(), targets.stride(0), None, 1))
         if out is None:
-            pot = torch.empty(n, dtype=torch.float64, device=xyz.device)
-            field = torch.empty((n, 3), dtype=torch.float64, device=xyz.device)
+            pot = torch.empty(m, dtype=torch.float64, device=xyz.device)
+            field = torch.empty((m, 3), dtype=torch.float64, device=xyz.device)
         else:
             pot, field = out
-        rc = lib().lfmm_fmm3d(ctx.handle, ctypes.byref(pts), None, float(eps), int(terms), int(tree_depth),
+        rc = lib().lfmm_fmm3d(ctx.handle, ctypes.byref(pts), tg, float(eps), int(terms), int(tree_depth),
                               int(leaf_max), MEM_DEVICE, flags, ctypes.c_void_p(pot.data_ptr()),
                               ctypes.c_void_p(field.data_ptr()), ctypes.byref(rep))
         _check(rc, "lfmm_fmm3d")
@@ -188,10 +196,16 @@ def fmm3d(xyz, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exc
     xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
     q = np.ascontiguousarray(q, dtype=np.float64)
     n = len(q)
-    pot = np.empty(n)
-    field = np.empty((n, 3))
     pts = Points3d(n, xyz.ctypes.data, 3, q.ctypes.data, 1)
-    rc = lib().lfmm_fmm3d(ctx.handle, ctypes.byref(pts), None, float(eps), int(terms), int(tree_depth),
+    tg = None
+    m = n
+    if targets is not None:
+        targets = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+        m = len(targets)
+        tg = ctypes.byref(Points3d(m, targets.ctypes.data, 3, None, 1))
+    pot = np.empty(m)
+    field = np.empty((m, 3))
+    rc = lib().lfmm_fmm3d(ctx.handle, ctypes.byref(pts), tg, float(eps), int(terms), int(tree_depth),
                           int(leaf_max), MEM_HOST, flags, pot.ctypes.data, field.ctypes.data, ctypes.byref(rep))
     _check(rc, "lfmm_fmm3d")
     return pot, field, rep.as_dict()
@@ -199,12 +213,14 @@ def fmm3d(xyz, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exc
 
 def tree3d(xyz, q, tree_depth, leaf_max=0, ctx=None):
     """The GPU Morton tree: dict with depth, level sizes, geometry (shift, scale),
-    pos_map (input -> sorted position), per-level keys and ranges."""
+    pos_map (input -> sorted position), per-level keys and ranges.  With
+    charges q, zero-charge particles follow the others inside each leaf."""
     ctx = ctx or default_context()
     xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
-    q = np.ascontiguousarray(q, dtype=np.float64)
-    n = len(q)
-    pts = Points3d(n, xyz.ctypes.data, 3, q.ctypes.data, 1)
+    n = len(xyz)
+    if q is not None:
+        q = np.ascontiguousarray(q, dtype=np.float64)
+    pts = Points3d(n, xyz.ctypes.data, 3, q.ctypes.data if q is not None else None, 1)
     sizes = np.zeros(32, dtype=np.int64)
     geom = np.zeros(4)
     pos_map = np.zeros(n, dtype=np.int64)

@@ -210,41 +210,91 @@ struct Tree {
     uint32_t max_leaf = 0;
 };
 
-// MortonTree constructor (src/cpu/morton_tree.cpp:286-493) on the GPU.
-int build_tree(lfmm_ctx* ctx, const lfmm_points3d* S, int memory, int depth_in, int leaf_max, Tree& T)
+// Device view of the particle set a solve works on.
+struct DevPoints {
+    const double* pos = nullptr;
+    int64_t ps = 3;
+    const double* q = nullptr;
+    int64_t qs = 1;
+    int64_t n = 0;
+};
+
+// Copies a host view to the device (keeping its strides).
+int upload_view(lfmm_ctx* ctx, const lfmm_points3d* S, DevPoints& P)
 {
     const int64_t n = S->n;
+    const int64_t ps = S->pos_stride, qs = S->q_stride;
+    hipStream_t st = ctx->stream;
+    Arena& A = ctx->arena;
+    P.n = n;
+    P.ps = ps;
+    P.qs = qs;
+    const size_t pbytes = sizeof(double) * (size_t)((n - 1) * ps + 3);
+    const bool shared = (S->q > S->pos && S->q < S->pos + (n - 1) * ps + 3 && ps == qs);
+    if (shared) {  // one interleaved block, e.g. the reference's particle AoS
+        const size_t span = sizeof(double) * (size_t)((n - 1) * ps + (S->q - S->pos) + 1);
+        const size_t bytes = std::max(span, pbytes);
+        double* d = take<double>(A, (int64_t)(bytes / 8) + 1);
+        if (!d) return LFMM_ERR_NOMEM;
+        LFMM_HIP(hipMemcpyAsync(d, S->pos, bytes, hipMemcpyHostToDevice, st));
+        P.q = d + (S->q - S->pos);
+        P.pos = d;
+        return LFMM_OK;
+    }
+    double* d = take<double>(A, (int64_t)(pbytes / 8) + 1);
+    if (!d) return LFMM_ERR_NOMEM;
+    LFMM_HIP(hipMemcpyAsync(d, S->pos, pbytes, hipMemcpyHostToDevice, st));
+    P.pos = d;
+    P.q = nullptr;
+    if (S->q) {
+        double* e = take<double>(A, (n - 1) * qs + 1);
+        if (!e) return LFMM_ERR_NOMEM;
+        LFMM_HIP(hipMemcpyAsync(e, S->q, sizeof(double) * (size_t)((n - 1) * qs + 1), hipMemcpyHostToDevice, st));
+        P.q = e;
+    }
+    return LFMM_OK;
+}
+
+// The particle set of a solve: the sources alone, or [targets (q = 0);
+// sources] as the reference's source/target constructor builds it
+// (src/cpu/multipole3d.cpp:1311-1319), packed as (x, y, z, q) records.
+int stage_inputs(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, int memory, DevPoints& P)
+{
+    DevPoints s, t;
+    if (memory == LFMM_MEM_HOST) {
+        int rc = upload_view(ctx, S, s);
+        if (rc != LFMM_OK) return rc;
+        if (Tg) {
+            rc = upload_view(ctx, Tg, t);
+            if (rc != LFMM_OK) return rc;
+        }
+    } else {
+        s = DevPoints{S->pos, S->pos_stride, S->q, S->q_stride, S->n};
+        if (Tg) t = DevPoints{Tg->pos, Tg->pos_stride, nullptr, 1, Tg->n};
+    }
+    if (!Tg) {
+        P = s;
+        return LFMM_OK;
+    }
+    const int64_t n = t.n + s.n;
+    double4* buf = take<double4>(ctx->arena, n);
+    if (!buf) return LFMM_ERR_NOMEM;
+    LFMM_HIP(pack_points(t.pos, t.ps, nullptr, 1, t.n, buf, ctx->stream));
+    LFMM_HIP(pack_points(s.pos, s.ps, s.q, s.qs, s.n, buf + t.n, ctx->stream));
+    P = DevPoints{(const double*)buf, 4, (const double*)buf + 3, 4, n};
+    return LFMM_OK;
+}
+
+// MortonTree constructor (src/cpu/morton_tree.cpp:286-493) on the GPU.
+int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, Tree& T)
+{
+    const int64_t n = DP.n;
     hipStream_t st = ctx->stream;
     Arena& A = ctx->arena;
     T.n = n;
-    // ---- inputs on the device
-    const double* dpos = S->pos;
-    const double* dq = S->q;
-    int64_t ps = S->pos_stride, qs = S->q_stride;
-    if (memory == LFMM_MEM_HOST) {
-        const size_t pbytes = sizeof(double) * (size_t)((n - 1) * ps + 3);
-        const bool shared = (S->q > S->pos && S->q < S->pos + (n - 1) * ps + 3 && ps == qs);
-        if (shared) {
-            const size_t span = sizeof(double) * (size_t)((n - 1) * ps + (S->q - S->pos) + 1);
-            const size_t bytes = std::max(span, pbytes);
-            double* d = take<double>(A, (int64_t)(bytes / 8) + 1);
-            if (!d) return LFMM_ERR_NOMEM;
-            LFMM_HIP(hipMemcpyAsync(d, S->pos, bytes, hipMemcpyHostToDevice, st));
-            dq = d + (S->q - S->pos);
-            dpos = d;
-        } else {
-            double* d = take<double>(A, (int64_t)(pbytes / 8) + 1);
-            if (!d) return LFMM_ERR_NOMEM;
-            LFMM_HIP(hipMemcpyAsync(d, S->pos, pbytes, hipMemcpyHostToDevice, st));
-            dpos = d;
-            if (S->q) {
-                double* e = take<double>(A, (n - 1) * qs + 1);
-                if (!e) return LFMM_ERR_NOMEM;
-                LFMM_HIP(hipMemcpyAsync(e, S->q, sizeof(double) * (size_t)((n - 1) * qs + 1), hipMemcpyHostToDevice, st));
-                dq = e;
-            }
-        }
-    }
+    const double* dpos = DP.pos;
+    const double* dq = DP.q;
+    const int64_t ps = DP.ps, qs = DP.qs;
     T.dpos = dpos;
     T.dq = dq;
     T.ps = ps;
@@ -297,11 +347,6 @@ int build_tree(lfmm_ctx* ctx, const lfmm_points3d* S, int memory, int depth_in, 
     uint32_t* order = v0;
     if (leaf_level > 0) LFMM_HIP(radix_sort(k0, v0, n, 3 * leaf_level, sws, st, &skeys, &order));
     T.order = order;
-    if (dq) {
-        T.src = take<double4>(A, n);
-        if (!T.src) return LFMM_ERR_NOMEM;
-        LFMM_HIP(gather_sources(dpos, ps, dq, qs, order, n, T.src, st));
-    }
 
     // ---- levels
     T.cnt.assign(depth, 0);
@@ -320,6 +365,13 @@ int build_tree(lfmm_ctx* ctx, const lfmm_points3d* S, int memory, int depth_in, 
         if (!T.lkeys[leaf_level] || !T.lbegin[leaf_level]) return LFMM_ERR_NOMEM;
         LFMM_HIP(run_compact(skeys, n, 0, flags_ws, excl_ws, scan_ws, T.lkeys[leaf_level], T.lbegin[leaf_level],
                              T.leaf_of, &T.cnt[leaf_level], st));
+    }
+    if (dq) {
+        // q != 0 first inside every leaf, then the sorted sources
+        LFMM_HIP(partition_leaves(T.lbegin[leaf_level], T.cnt[leaf_level], order, dq, qs, st));
+        T.src = take<double4>(A, n);
+        if (!T.src) return LFMM_ERR_NOMEM;
+        LFMM_HIP(gather_sources(dpos, ps, dq, qs, order, n, T.src, st));
     }
     for (int l = leaf_level - 1; l >= 0; --l) {
         const int64_t m = T.cnt[l + 1];
@@ -340,20 +392,25 @@ int build_tree(lfmm_ctx* ctx, const lfmm_points3d* S, int memory, int depth_in, 
     return LFMM_OK;
 }
 
-int fmm3d_self(lfmm_ctx* ctx, const lfmm_points3d* S, double eps, int terms_in, int depth_in, int leaf_max,
-               int memory, unsigned flags, double* pot_out, double* field_out, lfmm_report* rep)
+int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, double eps, int terms_in,
+                int depth_in, int leaf_max, int memory, unsigned flags, double* pot_out, double* field_out,
+                lfmm_report* rep)
 {
     const auto T0 = std::chrono::steady_clock::now();
     const bool timed = (flags & LFMM_FLAG_TIMINGS) != 0;
-    const int64_t n = S->n;
+    const int64_t n_out = Tg ? Tg->n : S->n;
     const int N = terms_3d(eps, terms_in);
     if (N < 1) return LFMM_ERR_TERMS;
     const int NX = tri(N);
     hipStream_t st = ctx->stream;
     Arena& A = ctx->arena;
     A.reset();
+    DevPoints DP;
+    int rc = stage_inputs(ctx, S, Tg, memory, DP);
+    if (rc != LFMM_OK) return rc;
+    const int64_t n = DP.n;
     Tree T;
-    int rc = build_tree(ctx, S, memory, depth_in, leaf_max, T);
+    rc = build_tree(ctx, DP, depth_in, leaf_max, T);
     if (rc != LFMM_OK) return rc;
     const int depth = T.depth;
     const int leaf_level = depth - 1;
@@ -430,8 +487,8 @@ int fmm3d_self(lfmm_ctx* ctx, const lfmm_points3d* S, double eps, int terms_in, 
     double* dpot = pot_out;
     double* dfield = field_out;
     if (memory == LFMM_MEM_HOST) {
-        dpot = pot_out ? take<double>(A, n) : nullptr;
-        dfield = field_out ? take<double>(A, 3 * n) : nullptr;
+        dpot = pot_out ? take<double>(A, n_out) : nullptr;
+        dfield = field_out ? take<double>(A, 3 * n_out) : nullptr;
         if ((pot_out && !dpot) || (field_out && !dfield)) return LFMM_ERR_NOMEM;
     }
     LeafArgs la{};
@@ -447,6 +504,7 @@ int fmm3d_self(lfmm_ctx* ctx, const lfmm_points3d* S, double eps, int terms_in, 
     la.knm = DT->knm;
     la.perm = order;
     la.n = n;
+    la.n_out = n_out;
     la.exclude_self = (flags & LFMM_FLAG_EXCLUDE_SELF) ? 1 : 0;
     la.pot = dpot;
     la.field = dfield;
@@ -455,8 +513,8 @@ int fmm3d_self(lfmm_ctx* ctx, const lfmm_points3d* S, double eps, int terms_in, 
     LFMM_HIP(launch_leaf(N, far, la, st));
     if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
     if (memory == LFMM_MEM_HOST) {
-        if (pot_out) LFMM_HIP(hipMemcpyAsync(pot_out, dpot, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-        if (field_out) LFMM_HIP(hipMemcpyAsync(field_out, dfield, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
+        if (pot_out) LFMM_HIP(hipMemcpyAsync(pot_out, dpot, sizeof(double) * n_out, hipMemcpyDeviceToHost, st));
+        if (field_out) LFMM_HIP(hipMemcpyAsync(field_out, dfield, sizeof(double) * 3 * n_out, hipMemcpyDeviceToHost, st));
     }
     if (timed || memory == LFMM_MEM_HOST) LFMM_HIP(hipStreamSynchronize(st));
     LFMM_HIP(hipGetLastError());
@@ -581,13 +639,28 @@ int lfmm_fmm3d(lfmm_ctx* c, const lfmm_points3d* sources, const lfmm_points3d* t
     if (terms != LFMM_AUTO && terms < 1) return LFMM_ERR_TERMS;
     if (tree_depth != LFMM_AUTO && (tree_depth < 1 || tree_depth > kMaxDepth)) return LFMM_ERR_DEPTH;
     if (!(eps > 0.0) && terms == LFMM_AUTO) return LFMM_ERR_INVALID;
-    if (targets && targets->n > 0) return LFMM_ERR_UNSUPPORTED;
+    if (targets && !valid_points(targets, false)) return LFMM_ERR_INVALID;
+    const lfmm_points3d* tg = (targets && targets->n > 0) ? targets : nullptr;
+    if (tg && sources->n + tg->n >= ((int64_t)1 << 31)) return LFMM_ERR_INVALID;
     if (report) std::memset(report, 0, sizeof(*report));
-    if (sources->n == 0) return LFMM_OK;
+    if ((tg ? tg->n : sources->n) == 0) return LFMM_OK;
+    if (sources->n == 0) {  // no sources: every output is zero
+        std::lock_guard<std::mutex> g(c->mu);
+        if (hipSetDevice(c->device) != hipSuccess) return LFMM_ERR_HIP;
+        const size_t nt = (size_t)tg->n;
+        if (memory == LFMM_MEM_HOST) {
+            if (potential) std::memset(potential, 0, nt * sizeof(double));
+            if (field) std::memset(field, 0, 3 * nt * sizeof(double));
+        } else {
+            if (potential && hipMemsetAsync(potential, 0, nt * sizeof(double), c->stream) != hipSuccess) return LFMM_ERR_HIP;
+            if (field && hipMemsetAsync(field, 0, 3 * nt * sizeof(double), c->stream) != hipSuccess) return LFMM_ERR_HIP;
+        }
+        return LFMM_OK;
+    }
     std::lock_guard<std::mutex> g(c->mu);
     if (hipSetDevice(c->device) != hipSuccess) return LFMM_ERR_HIP;
     try {
-        return fmm3d_self(c, sources, eps, terms, tree_depth, leaf_max, memory, flags, potential, field, report);
+        return fmm3d_solve(c, sources, tg, eps, terms, tree_depth, leaf_max, memory, flags, potential, field, report);
     } catch (const std::bad_alloc&) {
         return LFMM_ERR_NOMEM;
     } catch (...) {
@@ -605,9 +678,10 @@ int lfmm_tree3d(lfmm_ctx* c, const lfmm_points3d* particles, int tree_depth, int
     try {
         c->arena.reset();
         Tree T;
-        lfmm_points3d P = *particles;
-        P.q = nullptr;  // the tree needs positions only
-        int rc = build_tree(c, &P, LFMM_MEM_HOST, tree_depth, leaf_max, T);
+        DevPoints DP;
+        int rc = upload_view(c, particles, DP);
+        if (rc != LFMM_OK) return rc;
+        rc = build_tree(c, DP, tree_depth, leaf_max, T);
         if (rc != LFMM_OK) return rc;
         hipStream_t st = c->stream;
         const int64_t n = T.n;

@@ -52,6 +52,7 @@ struct LeafArgs {
     const double* knm;
     const uint32_t* perm;      // sorted -> input index
     int64_t n;
+    int64_t n_out;             // outputs for input indices < n_out only (targets first)
     int exclude_self;
     double* pot;
     double* field;

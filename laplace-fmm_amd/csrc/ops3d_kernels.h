@@ -536,6 +536,8 @@ __global__ void __launch_bounds__(256) k_leaf(LeafArgs a, const double2* __restr
 {
     const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= a.n) return;
+    const int64_t o = a.perm[s];
+    if (o >= a.n_out) return;  // a source of a source/target solve: no output
     const int leaf = a.leaf_of[s];
     const uint64_t key = a.leaf_key[leaf];
     const uint32_t ix = compact3(key >> 2), iy = compact3(key >> 1), iz = compact3(key);
@@ -577,7 +579,6 @@ __global__ void __launch_bounds__(256) k_leaf(LeafArgs a, const double2* __restr
             fz = fma(dzz, qi3, fz);
         }
     }
-    const int64_t o = a.perm[s];
     if (a.pot) a.pot[o] = ph;
     if (a.field) {
         a.field[3 * o] = fx;

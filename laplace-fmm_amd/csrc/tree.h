@@ -21,6 +21,11 @@ int64_t sort_ws_bytes(int64_t n);
 // in *keys_res / *vals_res (either the inputs or the workspace).
 hipError_t radix_sort(uint64_t* keys, uint32_t* vals, int64_t n, int bits, void* ws, hipStream_t st,
                       uint64_t** keys_res, uint32_t** vals_res);
+// (x, y, z, q) records from a strided view (q = 0 when q is null).
+hipError_t pack_points(const double* pos, int64_t ps, const double* q, int64_t qs, int64_t n, double4* out, hipStream_t st);
+// Per-leaf source/target partition of the sorted order (q != 0 first).
+hipError_t partition_leaves(const int32_t* begin, int64_t nleaf, uint32_t* order, const double* q, int64_t qs,
+                            hipStream_t st);
 hipError_t gather_sources(const double* pos, int64_t pstride, const double* q, int64_t qstride, const uint32_t* order,
                           int64_t n, double4* src, hipStream_t st);
 // Run-length compaction of sorted keys at shift sh.  Synchronises (reads the count).

@@ -332,6 +332,40 @@ __global__ void __launch_bounds__(kRsThreads) k_rs_scatter(const uint64_t* __res
     }
 }
 
+// ------------------------------------------------------------------ inputs
+// Packs a strided point view into (x, y, z, q) records at out[offset + i]
+// (q = 0 when no charges are given: the reference marks targets by q = 0,
+// src/examples/main.cpp:125-126).
+__global__ void __launch_bounds__(256) k_pack_points(const double* __restrict__ pos, int64_t ps,
+                                                     const double* __restrict__ q, int64_t qs, int64_t n,
+                                                     double4* __restrict__ out)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = make_double4(pos[i * ps], pos[i * ps + 1], pos[i * ps + 2], q ? q[i * qs] : 0.0);
+}
+
+// Source/target partition inside each leaf (src/cpu/morton_tree.cpp:437-452):
+// a forward scan moves every q != 0 particle to the front of its leaf by
+// swapping, so sources keep their order and the q = 0 targets follow.
+__global__ void __launch_bounds__(256) k_partition_leaves(const int32_t* __restrict__ begin, int nleaf,
+                                                          uint32_t* __restrict__ order, const double* __restrict__ q,
+                                                          int64_t qs)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nleaf) return;
+    const int b = begin[l], e = begin[l + 1];
+    int t = b;
+    for (int i = b; i < e; ++i) {
+        const uint32_t oi = order[i];
+        if (q[(int64_t)oi * qs] != 0.0) {
+            order[i] = order[t];
+            order[t] = oi;
+            ++t;
+        }
+    }
+}
+
 // ------------------------------------------------------------------ gather
 // Sorted sources as (x, y, z, q) double4 plus the sorted->original map.
 __global__ void __launch_bounds__(256) k_gather(const double* __restrict__ pos, int64_t pstride,
@@ -547,6 +581,19 @@ hipError_t radix_sort(uint64_t* keys, uint32_t* vals, int64_t n, int bits, void*
     }
     *keys_res = ka;
     *vals_res = va;
+    return hipGetLastError();
+}
+
+hipError_t pack_points(const double* pos, int64_t ps, const double* q, int64_t qs, int64_t n, double4* out, hipStream_t st)
+{
+    if (n > 0) hipLaunchKernelGGL(k_pack_points, dim3(nblk(n, 256)), dim3(256), 0, st, pos, ps, q, qs, n, out);
+    return hipGetLastError();
+}
+
+hipError_t partition_leaves(const int32_t* begin, int64_t nleaf, uint32_t* order, const double* q, int64_t qs,
+                            hipStream_t st)
+{
+    hipLaunchKernelGGL(k_partition_leaves, dim3(nblk(nleaf, 256)), dim3(256), 0, st, begin, (int)nleaf, order, q, qs);
     return hipGetLastError();
 }
 

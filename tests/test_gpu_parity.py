@@ -160,11 +160,42 @@ def test_errors_are_reported(lfmm, request):
         assert e.value.code == code
     import ctypes
     pts = lfmm.Points3d(100, x.ctypes.data, 3, q.ctypes.data, 1)
-    tg = lfmm.Points3d(10, x.ctypes.data, 3, None, 1)
+    tg = lfmm.Points3d(10, None, 3, None, 1)  # targets without positions
     pot = np.empty(10)
     rc = lfmm.lib().lfmm_fmm3d(lfmm.default_context().handle, ctypes.byref(pts), ctypes.byref(tg), 1e-8, 5, 3, 0,
                                lfmm.MEM_HOST, 0, pot.ctypes.data, None, None)
-    assert rc == 8
+    assert rc == 1
+
+
+@pytest.mark.parametrize("depth,N", [(4, 9), (5, 12), (2, 6)])
+def test_source_target_mode_parity(lfmm, orc, request, depth, N):
+    """Targets as q = 0 particles of one tree (src/cpu/multipole3d.cpp:1311-1319)."""
+    case_name(request)
+    src, q = uniform_cube(9000, 40 + depth)
+    tgt, _ = blobs(3000, 50 + depth)
+    pot, field, _ = lfmm.fmm3d(src, q, terms=N, tree_depth=depth, targets=tgt)
+    with orc.exact_geometry():
+        op, of = orc.fmm3d(src, q, N, depth, tgt_xyz=tgt)
+    assert rel_l2(pot, op) <= 1e-11 and rel_l2(field, of) <= 1e-10
+    dp, df = orc.direct3d(src, q, tgt_xyz=tgt, include_self=True)
+    assert rel_l2(pot, dp) < 1e-3
+    # the device path agrees with the host path bit for bit
+    import torch
+    pd, fd, _ = lfmm.fmm3d(torch.from_numpy(src).cuda(), torch.from_numpy(q).cuda(), terms=N, tree_depth=depth,
+                           targets=torch.from_numpy(tgt).cuda())
+    torch.cuda.synchronize()
+    assert np.array_equal(pd.cpu().numpy(), pot) and np.array_equal(fd.cpu().numpy(), field)
+
+
+def test_tree_partition_of_zero_charges(lfmm, orc, request):
+    case_name(request)
+    x, q = uniform_cube(8000, 61)
+    q[::4] = 0.0
+    g = lfmm.tree3d(x, q, 4)
+    t = orc.Tree3d(x, q, 4)
+    assert np.array_equal(g["pos_map"], t.pos_map)
+    for l in range(4):
+        assert np.array_equal(g["keys"][l], t.levels[l]["keys"])
 
 
 def test_auto_parameters(lfmm, request):
