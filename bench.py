@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particles/sec (3D, 4M) at rel-L2 err ≤1e-6 vs FP64 direct; M2L MFMA util"
 FP64_PEAK_TFLOPS = 78.6
+FP16_PEAK_TFLOPS = 2500.0  # dense MFMA
 HBM_PEAK_GBS = 8000.0
 
 
@@ -37,6 +38,13 @@ def m2l_flops_per_op(N):
     nx2 = N * (N + 1) // 2
     rot = sum((n + 1) * (4 * n + 2) for n in range(N))
     return 2 * (6 * nx2 + rot) + 4 * sum(t * t for t in range(1, N + 1))
+
+
+def m2l16_flops_per_op(N):
+    """Algorithmic matrix-core flops of one split-FP16 M2L with N terms: the
+    dense N^2 x N^2 real operator times one expansion, three fp16 products
+    (hi*hi, hi*lo, lo*hi) per operand pair."""
+    return 3 * 2 * (N * N) ** 2
 
 
 def main():
@@ -52,6 +60,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=524288, help="particles in the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--mixed", action="store_true", help="M2L of levels >= 4 on the matrix cores (split FP16)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -76,7 +85,7 @@ def main():
 
     def step(timings=False):
         return lfmm.fmm3d(xd, qd, terms=args.terms, tree_depth=args.depth, ctx=ctx, out=(pot, field),
-                          timings=timings)[2]
+                          timings=timings, mixed=args.mixed)[2]
 
     for _ in range(args.warmup):
         step()
@@ -96,7 +105,7 @@ def main():
 
     # exact op counts (after the timed region)
     counts = lfmm.fmm3d(xd, qd, terms=args.terms, tree_depth=args.depth, ctx=ctx, out=(pot, field),
-                        timings=True, count_ops=True)[2]
+                        timings=True, count_ops=True, mixed=args.mixed)[2]
     m2l_ms = float(np.mean([r["t_m2l_kernels_ms"] for r in reports]))
     launches = reports[0]["m2l_launches"]
     leaf_ms = float(np.mean([r["t_leaf_kernel_ms"] for r in reports]))
@@ -107,16 +116,32 @@ def main():
     if os.path.exists(tf):
         try:
             t = json.load(open(tf))
-            if t.get("n") == args.n and t.get("terms") == args.terms and t.get("tree_depth") == args.depth:
+            if (t.get("n") == args.n and t.get("terms") == args.terms and t.get("tree_depth") == args.depth
+                    and bool(t.get("mixed", False)) == args.mixed):
                 traffic = t.get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
-    roofline = {"kernel": "k_rtr<N,M2L> (M2L+L2L, one launch per level)", "bound": "mfma",
-                "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
-                "m2l_ops": counts["m2l_ops"], "flops_per_op": m2l_flops_per_op(args.terms),
-                "launches": launches, "avg_launch_ms": round(m2l_ms / max(launches, 1), 4),
-                "m2l_kernels_ms_per_step": round(m2l_ms, 4), "precision": "fp64 VALU"}
+    if args.mixed:
+        # dominant kernel: the matrix-core M2L of levels >= 4
+        mx_ms = float(np.mean([r["t_m2l_mixed_ms"] for r in reports]))
+        mx_launches = reports[0]["m2l_mixed_launches"]
+        mx_ops = counts["m2l_ops_mixed"]
+        mx_ach = mx_ops * m2l16_flops_per_op(args.terms) / (mx_ms * 1e-3) / 1e12
+        roofline = {"kernel": "k_m2l16<N> (split-FP16 MFMA M2L, levels >= 4, one launch per level)", "bound": "mfma",
+                    "achieved": round(mx_ach, 3), "peak": FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(mx_ach / FP16_PEAK_TFLOPS, 4), "traffic": traffic,
+                    "m2l_ops": mx_ops, "flops_per_op": m2l16_flops_per_op(args.terms), "launches": mx_launches,
+                    "avg_launch_ms": round(mx_ms / max(mx_launches, 1), 4), "m2l_kernels_ms_per_step": round(mx_ms, 4),
+                    "precision": "fp16 hi/lo operands (22 bits), fp32 accumulation",
+                    "fp64_levels": {"m2l_ops": counts["m2l_ops"] - mx_ops, "ms_per_step": round(m2l_ms - mx_ms, 4)},
+                    "fp64_equivalent_tflops": round(mx_ops * m2l_flops_per_op(args.terms) / (mx_ms * 1e-3) / 1e12, 3)}
+    else:
+        roofline = {"kernel": "k_rtr<N,M2L> (M2L+L2L, one launch per level)", "bound": "mfma",
+                    "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
+                    "m2l_ops": counts["m2l_ops"], "flops_per_op": m2l_flops_per_op(args.terms),
+                    "launches": launches, "avg_launch_ms": round(m2l_ms / max(launches, 1), 4),
+                    "m2l_kernels_ms_per_step": round(m2l_ms, 4), "precision": "fp64 VALU"}
     # near-field kernel as HBM-bound figure: 32 B per source read + 32 B per target in/out, per neighbour list
     p2p_pairs = counts["p2p_pairs"]
     leaf = {"kernel": "k_leaf<N> (P2P+L2P)", "ms_per_step": round(leaf_ms, 4), "p2p_pairs": p2p_pairs,
@@ -162,11 +187,12 @@ def main():
 
     out = {"metric": METRIC, "value": value, "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "vs_baseline": None, "dtype": "f64+f16x2" if args.mixed else "f64", "data": "synthetic",
            "config": {"workload": "3D Laplace FMM, %d particles uniform in [0,1)^3, q~U[-1,1], p=%d (%d terms), "
                                   "uniform octree depth %d (leaves at level %d)" % (args.n, args.terms - 1, args.terms,
                                                                                      args.depth, args.depth - 1),
                       "n": args.n, "terms": args.terms, "tree_depth": args.depth, "max_leaf": counts["max_leaf"],
+                      "m2l": ("split-FP16 MFMA at levels >= 4, FP64 at levels 2-3" if args.mixed else "FP64"),
                       "parallelism": "single GPU"},
            "accuracy": accuracy, "roofline": roofline, "leaf_kernel": leaf, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)

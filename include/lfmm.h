@@ -60,6 +60,10 @@ typedef enum lfmm_memory {
                                       (adds stream synchronisations) */
 #define LFMM_FLAG_COUNT_OPS 4u     /* count M2L and near-field pair operations
                                       into the report (extra kernels) */
+#define LFMM_FLAG_MIXED 8u         /* M2L at levels >= 4 on the matrix cores in
+                                      split FP16 (22-bit operands, FP32 sums);
+                                      levels 2-3, L2L, P2M, M2M, L2P, P2P stay
+                                      FP64.  Adds ~1e-7 relative error.      */
 
 typedef struct lfmm_ctx lfmm_ctx;
 
@@ -86,12 +90,15 @@ typedef struct lfmm_report {
     int64_t p2p_pairs;      /* near-field pair evaluations (with LFMM_FLAG_COUNT_OPS, else -1) */
     double t_tree_ms, t_upward_ms, t_downward_ms, t_leaf_ms, t_total_ms;
     /* With LFMM_FLAG_TIMINGS: summed device durations (HIP events on the
-     * solve's stream) of the M2L(+L2L) kernels, their launch count, and of
-     * the fused P2P+L2P leaf kernel. */
+     * solve's stream) of all M2L kernels (FP64 M2L+L2L, and with
+     * LFMM_FLAG_MIXED the matrix-core M2L), their launch count, and of the
+     * fused P2P+L2P leaf kernel. */
     double t_m2l_kernels_ms;
     int32_t m2l_launches;
-    int32_t reserved0;
+    int32_t m2l_mixed_launches;  /* matrix-core M2L launches (LFMM_FLAG_MIXED) */
     double t_leaf_kernel_ms;
+    double t_m2l_mixed_ms;       /* matrix-core M2L kernels alone (timings)    */
+    int64_t m2l_ops_mixed;       /* M2L translations on the matrix cores (count_ops) */
 } lfmm_report;
 
 /* Create a context bound to a GPU (device index as HIP numbers them). */
@@ -154,6 +161,13 @@ int lfmm_tree3d(lfmm_ctx* ctx, const lfmm_points3d* particles, int tree_depth, i
  * negative status. */
 int lfmm_host_wigner_d(int j, double beta, double* out);
 int64_t lfmm_host_op_tables(int N, int which, void* out, int64_t capacity);
+/* lfmm_host_m2l_dense: the dense real M2L operator of direction `dir`
+ * (0..315, the order of table 5) for N terms, N^2 x N^2 row-major doubles.
+ * lfmm_host_m2l16_pack: its split-fp16 packing for all directions as the
+ * matrix-core kernel reads it (LFMM_FLAG_MIXED); returns the element count
+ * (uint16 bit patterns) and the scale *kinv, or a negative status. */
+int lfmm_host_m2l_dense(int N, int dir, double* out);
+int64_t lfmm_host_m2l16_pack(int N, uint16_t* out, int64_t capacity, double* kinv);
 
 /* Library version string. */
 const char* lfmm_version(void);

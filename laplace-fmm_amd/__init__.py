@@ -22,7 +22,7 @@ CSRC = os.path.join(_HERE, "csrc")
 FMM_AUTO = 2147483647
 MAX_TERMS_3D = 18
 MEM_HOST, MEM_DEVICE = 0, 1
-FLAG_EXCLUDE_SELF, FLAG_TIMINGS, FLAG_COUNT_OPS = 1, 2, 4
+FLAG_EXCLUDE_SELF, FLAG_TIMINGS, FLAG_COUNT_OPS, FLAG_MIXED = 1, 2, 4, 8
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "expansion terms out of range", 3: "tree depth out of range",
           4: "out of device memory", 5: "HIP runtime error", 6: "non-finite input", 7: "no usable GPU",
@@ -30,7 +30,8 @@ STATUS = {0: "ok", 1: "invalid argument", 2: "expansion terms out of range", 3: 
 
 # Every extern "C" entry point declared in include/lfmm.h.
 EXPORTS = ("lfmm_create", "lfmm_destroy", "lfmm_strerror", "lfmm_set_stream", "lfmm_trim", "lfmm_fmm3d",
-           "lfmm_tree3d", "lfmm_version", "lfmm_host_wigner_d", "lfmm_host_op_tables")
+           "lfmm_tree3d", "lfmm_version", "lfmm_host_wigner_d", "lfmm_host_op_tables",
+           "lfmm_host_m2l_dense", "lfmm_host_m2l16_pack")
 
 
 class LfmmError(RuntimeError):
@@ -50,7 +51,9 @@ class Report(ctypes.Structure):
                 ("t_tree_ms", ctypes.c_double), ("t_upward_ms", ctypes.c_double),
                 ("t_downward_ms", ctypes.c_double), ("t_leaf_ms", ctypes.c_double),
                 ("t_total_ms", ctypes.c_double), ("t_m2l_kernels_ms", ctypes.c_double),
-                ("m2l_launches", ctypes.c_int32), ("reserved0", ctypes.c_int32), ("t_leaf_kernel_ms", ctypes.c_double)]
+                ("m2l_launches", ctypes.c_int32), ("m2l_mixed_launches", ctypes.c_int32),
+                ("t_leaf_kernel_ms", ctypes.c_double), ("t_m2l_mixed_ms", ctypes.c_double),
+                ("m2l_ops_mixed", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -106,6 +109,10 @@ def lib():
         L.lfmm_host_wigner_d.restype = ctypes.c_int
         L.lfmm_host_op_tables.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64]
         L.lfmm_host_op_tables.restype = ctypes.c_int64
+        L.lfmm_host_m2l_dense.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        L.lfmm_host_m2l_dense.restype = ctypes.c_int
+        L.lfmm_host_m2l16_pack.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+        L.lfmm_host_m2l16_pack.restype = ctypes.c_int64
         _lib = L
     return _lib
 
@@ -156,17 +163,19 @@ def _is_torch_cuda(x):
 
 
 def fmm3d(xyz, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exclude_self=False,
-          timings=False, count_ops=False, ctx=None, out=None, targets=None):
+          timings=False, count_ops=False, ctx=None, out=None, targets=None, mixed=False):
     """3D FMM: potential (m,) and field (m,3) at every target.
 
     xyz (n,3) and q (n,) are the sources; targets (m,3) defaults to the
     sources themselves (m = n).  float64 numpy arrays take the host path,
     CUDA torch tensors the device path (outputs are then CUDA tensors, or
     written into `out=(pot, field)`).  Returns (pot, field, report_dict).
+    mixed=True runs the M2L of levels >= 4 on the matrix cores in split FP16
+    with FP32 accumulation (LFMM_FLAG_MIXED); everything else stays FP64.
     """
     ctx = ctx or default_context()
     flags = ((FLAG_EXCLUDE_SELF if exclude_self else 0) | (FLAG_TIMINGS if timings else 0) |
-             (FLAG_COUNT_OPS if count_ops else 0))
+             (FLAG_COUNT_OPS if count_ops else 0) | (FLAG_MIXED if mixed else 0))
     rep = Report()
     if _is_torch_cuda(xyz):
         import torch
@@ -269,6 +278,25 @@ def host_op_tables(N, which):
     if got != count:
         raise LfmmError(1, "lfmm_host_op_tables")
     return out
+
+
+def host_m2l_dense(N, direction):
+    """Dense real M2L operator (N^2, N^2) of one direction (index into dir_vec)."""
+    out = np.zeros((N * N, N * N))
+    _check(lib().lfmm_host_m2l_dense(int(N), int(direction), out.ctypes.data), "lfmm_host_m2l_dense")
+    return out
+
+
+def host_m2l16_pack(N):
+    """Split-fp16 operator fragments of all directions: (uint16 array, kinv)."""
+    kinv = ctypes.c_double(0.0)
+    count = lib().lfmm_host_m2l16_pack(int(N), None, 0, ctypes.byref(kinv))
+    if count < 0:
+        raise LfmmError(int(-count), "lfmm_host_m2l16_pack")
+    out = np.zeros(count, dtype=np.uint16)
+    if lib().lfmm_host_m2l16_pack(int(N), out.ctypes.data, count, ctypes.byref(kinv)) != count:
+        raise LfmmError(1, "lfmm_host_m2l16_pack")
+    return out, kinv.value
 
 
 class FastMultipole3d:

@@ -88,7 +88,12 @@ struct DevTables {
     void* block = nullptr;
     RtrTables rt{};
     const double* knm = nullptr;
+    uint4* m16 = nullptr;      // split-fp16 dense M2L operators (built on first mixed solve)
+    double m16_kinv = 0.0;
 };
+
+// First M2L level on the matrix cores: the two coarsest levels stay FP64.
+constexpr int kMixedFirstLevel = 4;
 
 struct Status {
     int code = LFMM_OK;
@@ -161,6 +166,29 @@ int get_tables(lfmm_ctx* ctx, int N, const DevTables** out)
     }
     auto res = ctx->tables.emplace(N, D);
     *out = &res.first->second;
+    return LFMM_OK;
+}
+
+// Split-fp16 dense M2L operators of N terms (m2l_dense.cpp), built once per context.
+int get_m16(lfmm_ctx* ctx, int N, const DevTables** out)
+{
+    DevTables& D = ctx->tables.at(N);
+    if (!D.m16) {
+        OpTables T;
+        build_op_tables(N, T);
+        std::vector<uint16_t> pk;
+        double kinv = 0.0;
+        pack_m2l16(T, pk, kinv);
+        uint4* p = nullptr;
+        if (hipMalloc(&p, pk.size() * 2) != hipSuccess) return LFMM_ERR_NOMEM;
+        if (hipMemcpy(p, pk.data(), pk.size() * 2, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(p);
+            return LFMM_ERR_HIP;
+        }
+        D.m16 = p;
+        D.m16_kinv = kinv;
+    }
+    *out = &D;
     return LFMM_OK;
 }
 
@@ -286,7 +314,9 @@ int stage_inputs(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg,
 }
 
 // MortonTree constructor (src/cpu/morton_tree.cpp:286-493) on the GPU.
-int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, Tree& T)
+// `partition` applies the q = 0 partition of every leaf (morton_tree.cpp:426-468);
+// a single-set solve skips it, since it evaluates every particle either way.
+int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, bool partition, Tree& T)
 {
     const int64_t n = DP.n;
     hipStream_t st = ctx->stream;
@@ -368,7 +398,7 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, T
     }
     if (dq) {
         // q != 0 first inside every leaf, then the sorted sources
-        LFMM_HIP(partition_leaves(T.lbegin[leaf_level], T.cnt[leaf_level], order, dq, qs, st));
+        if (partition) LFMM_HIP(partition_leaves(T.lbegin[leaf_level], T.cnt[leaf_level], order, dq, qs, st));
         T.src = take<double4>(A, n);
         if (!T.src) return LFMM_ERR_NOMEM;
         LFMM_HIP(gather_sources(dpos, ps, dq, qs, order, n, T.src, st));
@@ -410,7 +440,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     if (rc != LFMM_OK) return rc;
     const int64_t n = DP.n;
     Tree T;
-    rc = build_tree(ctx, DP, depth_in, leaf_max, T);
+    rc = build_tree(ctx, DP, depth_in, leaf_max, Tg != nullptr, T);
     if (rc != LFMM_OK) return rc;
     const int depth = T.depth;
     const int leaf_level = depth - 1;
@@ -457,10 +487,24 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         t_up = ms_since(T0) - t_tree;
     }
     size_t nev = 0;
-    int m2l_launches = 0;
+    int m2l_launches = 0, mixed_launches = 0;
+    std::vector<char> ev_mixed;  // per event pair: matrix-core M2L or not
+    const bool mixed = far && (flags & LFMM_FLAG_MIXED) && depth - 1 >= kMixedFirstLevel;
+    uint4* xp = nullptr;
+    uint32_t* xmax = nullptr;
+    if (mixed) {
+        rc = get_m16(ctx, N, &DT);
+        if (rc != LFMM_OK) return rc;
+        int64_t big = 0;
+        for (int l = kMixedFirstLevel; l < depth; ++l) big = std::max(big, cnt[l]);
+        const int64_t nr = (int64_t)N * N, pieces = (nr + 15) / 16 * 4;
+        xp = take<uint4>(A, (big + 1) * pieces);
+        xmax = take<uint32_t>(A, depth);
+        if (!xp || !xmax) return LFMM_ERR_NOMEM;
+        LFMM_HIP(fill_u32(xmax, depth, 0u, st));
+    }
     if (far) {
         for (int l = 2; l < depth; ++l) {
-            if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
             RtrLevel L{};
             L.level = l;
             L.num_parents = (int)cnt[l - 1];
@@ -472,8 +516,38 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             L.src = outer[l];
             L.parent_local = (l > 2) ? inner[l - 1] : nullptr;
             L.dst = inner[l];
-            LFMM_HIP(launch_rtr(N, 1, DT->rt, L, st));
-            if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+            if (mixed && l >= kMixedFirstLevel) {
+                // FP64 L2L into the level, then += the matrix-core M2L
+                M2L16Args m{};
+                m.kmat = DT->m16;
+                m.outer = outer[l];
+                m.xp = xp;
+                m.xmax = xmax + l;
+                m.class_dirs = DT->rt.class_dirs;
+                m.dir_vec = DT->rt.dir_vec;
+                m.level = l;
+                m.num_parents = L.num_parents;
+                m.child_begin = L.child_begin;
+                m.child_keys = L.child_keys;
+                m.lut = L.lut;
+                m.keys = L.keys;
+                m.size = L.size;
+                m.kinv = DT->m16_kinv;
+                m.dst = inner[l];
+                LFMM_HIP(launch_absmax((const double*)outer[l], cnt[l] * NX * 2, xmax + l, st));
+                LFMM_HIP(launch_m2l16_pack(N, m, st));
+                LFMM_HIP(launch_rtr(N, 2, DT->rt, L, st));
+                if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+                LFMM_HIP(launch_m2l16(N, m, st));
+                if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+                ev_mixed.push_back(1);
+                ++mixed_launches;
+            } else {
+                if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+                LFMM_HIP(launch_rtr(N, 1, DT->rt, L, st));
+                if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+                ev_mixed.push_back(0);
+            }
             ++m2l_launches;
         }
     }
@@ -524,15 +598,19 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         rep->num_leaves = cnt[leaf_level];
         rep->max_leaf = max_leaf;
         rep->m2l_ops = -1;
+        rep->m2l_ops_mixed = -1;
         rep->p2p_pairs = -1;
         rep->m2l_launches = m2l_launches;
+        rep->m2l_mixed_launches = mixed_launches;
         if (timed) {
-            double tm = 0.0;
+            double tm = 0.0, tx = 0.0;
             for (size_t i = 0; i + 1 < ev_leaf; i += 2) {
                 float ms = 0.f;
                 LFMM_HIP(hipEventElapsedTime(&ms, ctx->events[i], ctx->events[i + 1]));
                 tm += ms;
+                if (ev_mixed[i / 2]) tx += ms;
             }
+            rep->t_m2l_mixed_ms = tx;
             float ml = 0.f;
             LFMM_HIP(hipEventElapsedTime(&ml, ctx->events[ev_leaf], ctx->events[ev_leaf + 1]));
             rep->t_m2l_kernels_ms = tm;
@@ -544,9 +622,13 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         rep->t_total_ms = timed ? ms_since(T0) : 0.0;
         rep->t_leaf_ms = timed ? rep->t_total_ms - t_tree - t_up - t_down : 0.0;
         if (flags & LFMM_FLAG_COUNT_OPS) {  // exact op counts, after the solve
-            unsigned long long* cws = take<unsigned long long>(A, 2);
+            unsigned long long* cws = take<unsigned long long>(A, 2 + depth);
             if (!cws) return LFMM_ERR_NOMEM;
-            LFMM_HIP(count_ops(lkeys, lbegin, lut, cnt, cws, &rep->m2l_ops, &rep->p2p_pairs, st));
+            std::vector<int64_t> per(depth, 0);
+            LFMM_HIP(count_ops(lkeys, lbegin, lut, cnt, cws, &rep->m2l_ops, &rep->p2p_pairs, per.data(), st));
+            rep->m2l_ops_mixed = 0;
+            if (mixed)
+                for (int l = kMixedFirstLevel; l < depth; ++l) rep->m2l_ops_mixed += per[l];
         }
     }
     return LFMM_OK;
@@ -591,7 +673,10 @@ void lfmm_destroy(lfmm_ctx* c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     c->arena.release();
-    for (auto& kv : c->tables) (void)hipFree(kv.second.block);
+    for (auto& kv : c->tables) {
+        (void)hipFree(kv.second.block);
+        (void)hipFree(kv.second.m16);
+    }
     for (auto e : c->events) (void)hipEventDestroy(e);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
@@ -681,7 +766,7 @@ int lfmm_tree3d(lfmm_ctx* c, const lfmm_points3d* particles, int tree_depth, int
         DevPoints DP;
         int rc = upload_view(c, particles, DP);
         if (rc != LFMM_OK) return rc;
-        rc = build_tree(c, DP, tree_depth, leaf_max, T);
+        rc = build_tree(c, DP, tree_depth, leaf_max, true, T);
         if (rc != LFMM_OK) return rc;
         hipStream_t st = c->stream;
         const int64_t n = T.n;
@@ -745,6 +830,37 @@ int64_t lfmm_host_op_tables(int N, int which, void* out, int64_t capacity)
         }
         if (out && capacity >= count) std::memcpy(out, p, esz * (size_t)count);
         return count;
+    } catch (...) {
+        return -(int64_t)LFMM_ERR_NOMEM;
+    }
+}
+
+int lfmm_host_m2l_dense(int N, int dir, double* out)
+{
+    if (N < 1 || N > LFMM_MAX_TERMS_3D) return LFMM_ERR_TERMS;
+    if (dir < 0 || dir >= kNumM2LDirs || !out) return LFMM_ERR_INVALID;
+    try {
+        OpTables T;
+        build_op_tables(N, T);
+        m2l_dense_matrix(T, dir, out);
+        return LFMM_OK;
+    } catch (...) {
+        return LFMM_ERR_NOMEM;
+    }
+}
+
+int64_t lfmm_host_m2l16_pack(int N, uint16_t* out, int64_t capacity, double* kinv)
+{
+    if (N < 1 || N > LFMM_MAX_TERMS_3D) return -(int64_t)LFMM_ERR_TERMS;
+    try {
+        OpTables T;
+        build_op_tables(N, T);
+        std::vector<uint16_t> pk;
+        double ki = 0.0;
+        pack_m2l16(T, pk, ki);
+        if (kinv) *kinv = ki;
+        if (out && capacity >= (int64_t)pk.size()) std::memcpy(out, pk.data(), pk.size() * 2);
+        return (int64_t)pk.size();
     } catch (...) {
         return -(int64_t)LFMM_ERR_NOMEM;
     }

@@ -50,6 +50,14 @@ struct OpTables {
 
 void build_op_tables(int N, OpTables& T);
 
+// Dense real form of the M2L operator of direction `dir` (< kNumM2LDirs):
+// K[row * N^2 + col], rows and columns in m2l_real_index order.
+int m2l_real_index(int n, int m, int part);
+void m2l_dense_matrix(const OpTables& T, int dir, double* K);
+// All M2L operators scaled by 1/kinv (a power of two) and split into fp16
+// hi/lo A-operand fragments of the matrix-core M2L kernel.
+void pack_m2l16(const OpTables& T, std::vector<uint16_t>& out, double& kinv);
+
 // Small host helpers exposed for tests through the C ABI debug entry points.
 double host_knm(int n, int m);
 double host_anm(int n, int m);

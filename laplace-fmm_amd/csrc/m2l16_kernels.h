@@ -1,0 +1,252 @@
+// m2l16_kernels.h -- M2L on the matrix cores in split FP16 with FP32
+// accumulation (gfx950 v_mfma_f32_32x32x16_f16).
+//
+// For one level and one child class, the targets of a workgroup share their
+// 189 interaction directions (reference M2L :152-190 and Downward :328-349 of
+// src/cpu/multipole3d.cpp).  Per direction d the operator is the dense real
+// matrix K_d (N^2 x N^2, built on the host from the FP64 rotation and
+// translation tables, m2l_dense.cpp), so one direction step of a workgroup
+// is the GEMM  L[:, targets] += K_d X[:, sources(targets, d)].
+//
+// Precision: K_d and the source expansions X are scaled by powers of two and
+// split into fp16 hi + lo (22 significant bits).  Each product is
+//   hi*hi + hi*lo + lo*hi   (the lo*lo term, ~2^-22 relative, is dropped)
+// computed exactly on the matrix cores and accumulated in FP32 over all K and
+// all 189 directions.  The result is scaled back and added in FP64 to the
+// local expansion, which already holds the FP64 L2L of the parent.
+//
+// Operands (32x32x16 f16 MFMA; lane l, slot s in 0..7):
+//   A  row l % 32, K index 8 * (l / 32) + s     (operator, from global/L2)
+//   B  col l % 32, K index 8 * (l / 32) + s     (sources, staged in LDS)
+//   C  col l % 32, row 8 * (v / 4) + 4 * (l / 32) + v % 4, v = 0..15
+// A and B use the same (lane, slot) -> K assignment, so the sum over K does
+// not depend on how the hardware numbers K within an instruction.
+#pragma once
+#include "ops3d_kernels.h"  // xcd_local_block, lds_barrier
+
+namespace lfmm {
+
+typedef _Float16 m16_h8 __attribute__((ext_vector_type(8)));
+typedef float m16_acc __attribute__((ext_vector_type(16)));
+
+template <int N>
+struct M16 {
+    static constexpr int NR = N * N;            // real coordinates of an expansion
+    static constexpr int RT = (NR + 31) / 32;   // row tiles of 32
+    static constexpr int KC = (NR + 15) / 16;   // K chunks of 16
+    static constexpr int RW = (RT + 3) / 4;     // row tiles per wave (at most)
+    static constexpr int PIECES = KC * 4;       // 16-byte pieces of a packed source box
+};
+
+// Scale of a level's sources: the largest |coefficient| lands below 2^14.
+__device__ inline double m16_xscale(uint32_t maxbits)
+{
+    const float mx = __uint_as_float(maxbits);
+    if (!(mx > 0.0f)) return 1.0;
+    const int e = max((int)((maxbits >> 23) & 0xff) - 126, -100);  // mx < 2^e
+    return ldexp(1.0, 14 - e);
+}
+
+// Real coordinate r (< N^2) -> offset of the double inside [tri(N)] double2.
+__device__ inline int m16_coef_offset(int r)
+{
+    const int n = (int)sqrtf((float)r + 0.5f);
+    const int q = r - n * n;
+    const int m = (q + 1) >> 1;
+    const int part = (q != 0 && !(q & 1)) ? 1 : 0;
+    return 2 * (tri(n) + m) + part;
+}
+
+// Packs a level's normalised multipoles into the B operand layout:
+// [box][KC][part hi/lo][half][8 x fp16]; box `size` is all zeros.
+template <int N>
+__global__ void __launch_bounds__(256) k_m2l16_pack(const double2* __restrict__ outer, int size,
+                                                    const uint32_t* __restrict__ xmax, uint4* __restrict__ xp)
+{
+    using S = M16<N>;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (box, kc, half)
+    if (i >= (int64_t)(size + 1) * S::KC * 2) return;
+    const int half = (int)(i & 1);
+    const int kc = (int)((i >> 1) % S::KC);
+    const int64_t box = (i >> 1) / S::KC;
+    const double sc = m16_xscale(*xmax);
+    const double* src = (const double*)(outer + box * tri(N));
+    m16_h8 hv, lv;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int k = kc * 16 + half * 8 + s;
+        double x = 0.0;
+        if (box < size && k < S::NR) x = src[m16_coef_offset(k)] * sc;
+        const _Float16 h = (_Float16)x;
+        hv[s] = h;
+        lv[s] = (_Float16)(x - (double)h);
+    }
+    uint4* dst = xp + (box * S::KC + kc) * 4;
+    dst[half] = __builtin_bit_cast(uint4, hv);
+    dst[2 + half] = __builtin_bit_cast(uint4, lv);
+}
+
+template <int N, int CT>
+__global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restrict__ kmat, const uint4* __restrict__ xp,
+                                               const int* __restrict__ class_dirs, const int* __restrict__ dir_vec)
+{
+    using S = M16<N>;
+    constexpr int RT = S::RT, KC = S::KC, RW = S::RW, NR = S::NR, PIECES = S::PIECES;
+    constexpr int TC = 32 * CT;        // targets (columns) per workgroup
+    constexpr int TPC = 256 / TC;      // loader threads per column
+    constexpr int PPT = (PIECES + TPC - 1) / TPC;  // 16-byte pieces per loader thread
+    static_assert(256 % TC == 0, "loader split");
+    __shared__ uint4 bs[2][CT][KC][2][64];
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int blk = xcd_local_block(blockIdx.x, gridDim.x);
+    const int tile = blk >> 3, c = blk & 7;
+    const int side = 1 << a.level;
+
+    // target box of column `col`: the class-c child of parent tile * TC + col
+    auto target_of = [&](int col, uint32_t& x, uint32_t& y, uint32_t& z) -> int {
+        const int P = tile * TC + col;
+        int t = -1;
+        if (P < a.num_parents) {
+            const int cb = a.child_begin[P], ce = a.child_begin[P + 1];
+            for (int i = cb; i < ce; ++i)
+                if ((int)(a.child_keys[i] & 7u) == c) t = i;
+        }
+        if (t >= 0) {
+            const uint64_t k = a.child_keys[t];
+            x = compact3(k >> 2);
+            y = compact3(k >> 1);
+            z = compact3(k);
+        }
+        return t;
+    };
+
+    // ---- B staging: loader thread (column lcol, part lq) moves PPT pieces
+    const int lcol = tid / TPC, lq = tid % TPC;
+    uint32_t lx = 0, ly = 0, lz = 0;
+    const int ltgt = target_of(lcol, lx, ly, lz);
+    uint4 nb[PPT];
+    auto load_b = [&](int d) {
+        const int dir = class_dirs[c * kDirsPerClass + d];
+        int box = a.size;  // the zero box
+        if (ltgt >= 0) {
+            const int sx = (int)lx + dir_vec[4 * dir], sy = (int)ly + dir_vec[4 * dir + 1],
+                      sz = (int)lz + dir_vec[4 * dir + 2];
+            if (sx >= 0 && sy >= 0 && sz >= 0 && sx < side && sy < side && sz < side) {
+                const int s = find_box(morton3(sx, sy, sz), a.lut, a.keys, a.size);
+                if (s >= 0) box = s;
+            }
+        }
+        const uint4* p = xp + (int64_t)box * PIECES;
+#pragma unroll
+        for (int i = 0; i < PPT; ++i)
+            if (PIECES % TPC == 0 || lq + TPC * i < PIECES) nb[i] = p[lq + TPC * i];
+    };
+    auto store_b = [&](int buf) {
+        const int ct = lcol >> 5, cl = lcol & 31;
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            const int pc = lq + TPC * i;
+            if (PIECES % TPC == 0 || pc < PIECES) bs[buf][ct][pc >> 2][(pc >> 1) & 1][(pc & 1) * 32 + cl] = nb[i];
+        }
+    };
+
+    m16_acc acc[RW][CT];
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[r][ct][v] = 0.0f;
+
+    load_b(0);
+    store_b(0);
+    lds_barrier();
+    for (int d = 0; d < kDirsPerClass; ++d) {
+        const int buf = d & 1;
+        const int dir = class_dirs[c * kDirsPerClass + d];
+        const uint4* ka = kmat + (int64_t)dir * (RT * KC * 2 * 64) + lane;
+        uint4 ah[RW][KC], al[RW][KC];
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            const int rt = w + 4 * r;
+            if (rt < RT) {
+#pragma unroll
+                for (int kc = 0; kc < KC; ++kc) {
+                    ah[r][kc] = ka[((rt * KC + kc) * 2) * 64];
+                    al[r][kc] = ka[((rt * KC + kc) * 2 + 1) * 64];
+                }
+            }
+        }
+        if (d + 1 < kDirsPerClass) load_b(d + 1);  // next step's sources, in flight under the MFMAs
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) {
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+                const m16_h8 bh = __builtin_bit_cast(m16_h8, bs[buf][ct][kc][0][lane]);
+                const m16_h8 bl = __builtin_bit_cast(m16_h8, bs[buf][ct][kc][1][lane]);
+#pragma unroll
+                for (int r = 0; r < RW; ++r) {
+                    if (w + 4 * r < RT) {
+                        const m16_h8 xh = __builtin_bit_cast(m16_h8, ah[r][kc]);
+                        const m16_h8 xl = __builtin_bit_cast(m16_h8, al[r][kc]);
+                        acc[r][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bh, acc[r][ct], 0, 0, 0);
+                        acc[r][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bl, acc[r][ct], 0, 0, 0);
+                        acc[r][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xl, bh, acc[r][ct], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        if (d + 1 < kDirsPerClass) store_b(buf ^ 1);
+        lds_barrier();
+    }
+
+    // ---- epilogue: L += acc / (scale of K * scale of X), in FP64
+    const double inv = a.kinv / m16_xscale(*a.xmax);
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+        uint32_t tx, ty, tz;
+        const int t = target_of(ct * 32 + (lane & 31), tx, ty, tz);
+        if (t < 0) continue;
+        double* out = (double*)(a.dst + (int64_t)t * tri(N));
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            const int rt = w + 4 * r;
+            if (rt >= RT) continue;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = rt * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                if (row < NR) {
+                    const int o = m16_coef_offset(row);
+                    out[o] += (double)acc[r][ct][v] * inv;
+                }
+            }
+        }
+    }
+}
+
+// Targets per workgroup = 32 * m16_ct(N): two column tiles while the
+// double-buffered source stage (64 KiB per tile at N = 16) fits the LDS.
+constexpr int m16_ct(int N) { return N <= 16 ? 2 : 1; }
+
+template <int N>
+hipError_t do_m2l16_pack(const M2L16Args& a, hipStream_t st)
+{
+    const int64_t packs = (int64_t)(a.size + 1) * M16<N>::KC * 2;
+    hipLaunchKernelGGL((k_m2l16_pack<N>), dim3((unsigned)((packs + 255) / 256)), dim3(256), 0, st, a.outer, a.size,
+                       a.xmax, a.xp);
+    return hipGetLastError();
+}
+
+template <int N>
+hipError_t do_m2l16(const M2L16Args& a, hipStream_t st)
+{
+    const int tc = 32 * m16_ct(N);
+    const int tiles = (a.num_parents + tc - 1) / tc;
+    if (tiles > 0)
+        hipLaunchKernelGGL((k_m2l16<N, m16_ct(N)>), dim3(tiles * 8), dim3(256), 0, st, a, a.kmat, (const uint4*)a.xp,
+                           a.class_dirs, a.dir_vec);
+    return hipGetLastError();
+}
+
+}  // namespace lfmm

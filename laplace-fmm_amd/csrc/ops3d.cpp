@@ -7,6 +7,8 @@ namespace lfmm {
 template <int N> hipError_t do_p2m(const P2MArgs& a, hipStream_t st);
 template <int N> hipError_t do_rtr(int kind, const RtrTables& T, const RtrLevel& L, hipStream_t st);
 template <int N> hipError_t do_leaf(bool local, const LeafArgs& a, hipStream_t st);
+template <int N> hipError_t do_m2l16_pack(const M2L16Args& a, hipStream_t st);
+template <int N> hipError_t do_m2l16(const M2L16Args& a, hipStream_t st);
 
 #define LFMM_FOR_TERMS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18)
 
@@ -34,6 +36,26 @@ hipError_t launch_leaf(int N, bool local, const LeafArgs& a, hipStream_t st)
 {
     switch (N) {
 #define X(n) case n: return do_leaf<n>(local, a, st);
+        LFMM_FOR_TERMS(X)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_m2l16_pack(int N, const M2L16Args& a, hipStream_t st)
+{
+    switch (N) {
+#define X(n) case n: return do_m2l16_pack<n>(a, st);
+        LFMM_FOR_TERMS(X)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_m2l16(int N, const M2L16Args& a, hipStream_t st)
+{
+    switch (N) {
+#define X(n) case n: return do_m2l16<n>(a, st);
         LFMM_FOR_TERMS(X)
 #undef X
     default: return hipErrorInvalidValue;

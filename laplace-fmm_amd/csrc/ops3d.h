@@ -58,8 +58,31 @@ struct LeafArgs {
     double* field;
 };
 
+// Matrix-core M2L of one level (m2l16_kernels.h).
+struct M2L16Args {
+    const uint4* kmat;         // split-fp16 operator fragments [dir][RT][KC][2][64]
+    const double2* outer;      // normalised multipoles of the level
+    uint4* xp;                 // packed sources [size + 1][KC][2][2] (box `size` is zero)
+    const uint32_t* xmax;      // largest |coefficient| of the level (float bits)
+    const int* class_dirs;
+    const int* dir_vec;
+    int level;                 // target level
+    int num_parents;           // boxes at level - 1
+    const int32_t* child_begin;
+    const uint64_t* child_keys;
+    const int32_t* lut;        // dense key -> box table at `level`, or null
+    const uint64_t* keys;
+    int size;                  // boxes at `level`
+    double kinv;               // 1 / scale of the packed operators
+    double2* dst;              // locals of the level (+= M2L)
+};
+
 hipError_t launch_p2m(int N, const P2MArgs& a, hipStream_t st);
 hipError_t launch_rtr(int N, int kind, const RtrTables& T, const RtrLevel& L, hipStream_t st);
 hipError_t launch_leaf(int N, bool local, const LeafArgs& a, hipStream_t st);
+hipError_t launch_m2l16_pack(int N, const M2L16Args& a, hipStream_t st);
+hipError_t launch_m2l16(int N, const M2L16Args& a, hipStream_t st);
+// *out = max(*out, largest |v[i]|) as float bits (m2l16.hip)
+hipError_t launch_absmax(const double* v, int64_t count, uint32_t* out, hipStream_t st);
 
 }  // namespace lfmm

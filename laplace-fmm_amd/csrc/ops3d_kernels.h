@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(256) k_p2m(P2MArgs a, const double* __restrict
 }
 
 // ===================================================================== rotate-translate-rotate
-enum { KIND_M2M = 0, KIND_M2L = 1 };
+enum { KIND_M2M = 0, KIND_M2L = 1, KIND_L2L = 2 };  // KIND_L2L: the L2L step of KIND_M2L alone
 
 // Targets per workgroup: 64 (one per lane) while the FP64 op state of 64
 // targets fits the 160 KiB LDS, fewer lanes above that (N = 18).
@@ -241,7 +241,7 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
     const bool lok = lane < TW;
     const int blk = xcd_local_block(blockIdx.x, gridDim.x);
     int tile, c;
-    if constexpr (KIND == KIND_M2L) {
+    if constexpr (KIND != KIND_M2M) {
         tile = blk >> 3;
         c = blk & 7;
     } else {
@@ -257,7 +257,7 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
     }
     int tgt = -1;
     uint32_t tx = 0, ty = 0, tz = 0;
-    if constexpr (KIND == KIND_M2L) {
+    if constexpr (KIND != KIND_M2M) {
         for (int i = cb; i < ce; ++i)
             if ((int)(L.child_keys[i] & 7u) == c) tgt = i;
         if (tgt >= 0) {
@@ -274,8 +274,8 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
 #pragma unroll
     for (int i = 0; i < NX; ++i) acc[i] = make_double2(0.0, 0.0);
 
-    const int has_l2l = (KIND == KIND_M2L && L.parent_local) ? 1 : 0;
-    const int ndir = (KIND == KIND_M2M) ? 8 : (kDirsPerClass + has_l2l);
+    const int has_l2l = (KIND == KIND_L2L || (KIND == KIND_M2L && L.parent_local)) ? 1 : 0;
+    const int ndir = (KIND == KIND_M2M) ? 8 : (KIND == KIND_L2L ? 1 : kDirsPerClass + has_l2l);
 
     auto step_of = [&](int d) {
         RtrStep s;
@@ -360,6 +360,8 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
             const double* __restrict__ tt = Ttrans + (int64_t)tsel * TS;
             if constexpr (KIND == KIND_M2M) {
                 rtr_phase_b<N, W, TR_M2M>(tt, lds, lane);
+            } else if constexpr (KIND == KIND_L2L) {
+                rtr_phase_b<N, W, TR_L2L>(tt, lds, lane);
             } else {
                 if (cur.tk == TR_L2L) rtr_phase_b<N, W, TR_L2L>(tt, lds, lane);
                 else rtr_phase_b<N, W, TR_M2L>(tt, lds, lane);
@@ -603,6 +605,9 @@ hipError_t do_rtr(int kind, const RtrTables& T, const RtrLevel& L, hipStream_t s
     if (tiles == 0) return hipSuccess;
     if (kind == KIND_M2M)
         hipLaunchKernelGGL((k_rtr<N, KIND_M2M>), dim3(tiles), dim3(256), 0, st, T.rot, T.ephi, T.trans, T.dir_theta,
+                           T.dir_trans, T.dir_vec, T.class_dirs, L);
+    else if (kind == KIND_L2L)
+        hipLaunchKernelGGL((k_rtr<N, KIND_L2L>), dim3(tiles * 8), dim3(256), 0, st, T.rot, T.ephi, T.trans, T.dir_theta,
                            T.dir_trans, T.dir_vec, T.class_dirs, L);
     else
         hipLaunchKernelGGL((k_rtr<N, KIND_M2L>), dim3(tiles * 8), dim3(256), 0, st, T.rot, T.ephi, T.trans, T.dir_theta,

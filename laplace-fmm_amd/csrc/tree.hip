@@ -631,22 +631,36 @@ hipError_t build_lut(const uint64_t* ukeys, int64_t count, int level, int32_t* l
 
 hipError_t count_ops(const std::vector<uint64_t*>& keys, const std::vector<int32_t*>& begin,
                      const std::vector<int32_t*>& lut, const std::vector<int64_t>& cnt, unsigned long long* ws,
-                     int64_t* m2l_ops, int64_t* p2p_pairs, hipStream_t st)
+                     int64_t* m2l_ops, int64_t* p2p_pairs, int64_t* m2l_per_level, hipStream_t st)
 {
     const int depth = (int)cnt.size();
-    hipError_t e = hipMemsetAsync(ws, 0, 2 * sizeof(unsigned long long), st);
+    hipError_t e = hipMemsetAsync(ws, 0, (2 + depth) * sizeof(unsigned long long), st);
     if (e != hipSuccess) return e;
     for (int l = 2; l < depth; ++l)
-        hipLaunchKernelGGL(k_count_m2l, dim3(nblk(cnt[l], 256)), dim3(256), 0, st, keys[l], (int)cnt[l], l, lut[l], ws);
+        hipLaunchKernelGGL(k_count_m2l, dim3(nblk(cnt[l], 256)), dim3(256), 0, st, keys[l], (int)cnt[l], l, lut[l],
+                           ws + 2 + l);
     const int L = depth - 1;
     hipLaunchKernelGGL(k_count_p2p, dim3(nblk(cnt[L], 256)), dim3(256), 0, st, keys[L], begin[L], (int)cnt[L], L,
                        L >= 2 ? lut[L] : nullptr, ws + 1);
-    unsigned long long h[2];
-    e = hipMemcpyAsync(h, ws, sizeof(h), hipMemcpyDeviceToHost, st);
+    std::vector<unsigned long long> h(2 + depth);
+    e = hipMemcpyAsync(h.data(), ws, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    *m2l_ops = (int64_t)h[0];
+    int64_t total = 0;
+    for (int l = 0; l < depth; ++l) {
+        total += (int64_t)h[2 + l];
+        if (m2l_per_level) m2l_per_level[l] = (int64_t)h[2 + l];
+    }
+    *m2l_ops = total;
     *p2p_pairs = (int64_t)h[1];
     return e;
+}
+
+hipError_t fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_fill_i32, dim3((int)std::min<int64_t>(4096, nblk(n, 256))), dim3(256), 0, st, (int32_t*)p, n,
+                           (int32_t)v);
+    return hipGetLastError();
 }
 
 hipError_t max_run(const int32_t* begin, int64_t count, uint32_t* maxv_dev, uint32_t* out_host, hipStream_t st)
