@@ -81,6 +81,12 @@ typedef struct lfmm_points3d {
     int64_t q_stride;
 } lfmm_points3d;
 
+/* 2D particle view: the same layout with two coordinates per point,
+ * (pos[i*pos_stride], pos[i*pos_stride+1]).  The reference's 32-byte particle
+ * {std::complex<double> center; double q; size_t morton} (src/common/utils.h:
+ * 26-32, 149-151) uses pos = &p[0].center, q = &p[0].q and strides 4 and 4. */
+typedef lfmm_points3d lfmm_points2d;
+
 typedef struct lfmm_report {
     int32_t terms;          /* N: expansion terms used (order p = N - 1)  */
     int32_t tree_depth;     /* number of levels; leaves at tree_depth - 1 */
@@ -149,6 +155,28 @@ int lfmm_fmm3d(lfmm_ctx* ctx, const lfmm_points3d* sources, const lfmm_points3d*
  *                  range (particles at the leaves, children above) plus the end
  */
 int lfmm_tree3d(lfmm_ctx* ctx, const lfmm_points3d* particles, int tree_depth, int leaf_max,
+                int32_t* depth_out, int64_t* level_sizes, double* geom, int64_t* pos_map,
+                uint64_t* keys, int64_t* begins);
+
+/*
+ * 2D Laplace FMM with complex-series expansions; replaces the constructors of
+ * fmm::gpu::FastMultipole (src/cuda/multipole.h:11-12; CPU twin
+ * src/cpu/multipole.h:13-15).  Arguments as lfmm_fmm3d, except:
+ *   terms: LFMM_AUTO follows src/cpu/multipole.cpp:478-481, clamped to 30;
+ *   tree_depth: LFMM_AUTO follows src/cpu/multipole.cpp:483-485 (at least 3);
+ *   potential: n_out doubles, sum q log|z - z_j| (self pair clamped at 1e-4);
+ *   field: n_out*2 doubles, (re, im) of the complex force sum q / (z - z_j),
+ *     the reference's std::complex<double> forces;
+ *   LFMM_FLAG_MIXED is not supported (LFMM_ERR_UNSUPPORTED).
+ */
+int lfmm_fmm2d(lfmm_ctx* ctx, const lfmm_points2d* sources, const lfmm_points2d* targets,
+               double eps, int terms, int tree_depth, int leaf_max,
+               int memory, unsigned flags,
+               double* potential, double* field, lfmm_report* report);
+
+/* The Morton quadtree of a 2D particle set, as lfmm_tree3d (reference
+ * MortonTree<2, point2d, double>); keys interleave x << 1 | y. */
+int lfmm_tree2d(lfmm_ctx* ctx, const lfmm_points2d* particles, int tree_depth, int leaf_max,
                 int32_t* depth_out, int64_t* level_sizes, double* geom, int64_t* pos_map,
                 uint64_t* keys, int64_t* begins);
 

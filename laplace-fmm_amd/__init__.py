@@ -30,7 +30,7 @@ STATUS = {0: "ok", 1: "invalid argument", 2: "expansion terms out of range", 3: 
 
 # Every extern "C" entry point declared in include/lfmm.h.
 EXPORTS = ("lfmm_create", "lfmm_destroy", "lfmm_strerror", "lfmm_set_stream", "lfmm_trim", "lfmm_fmm3d",
-           "lfmm_tree3d", "lfmm_version", "lfmm_host_wigner_d", "lfmm_host_op_tables",
+           "lfmm_tree3d", "lfmm_fmm2d", "lfmm_tree2d", "lfmm_version", "lfmm_host_wigner_d", "lfmm_host_op_tables",
            "lfmm_host_m2l_dense", "lfmm_host_m2l16_pack")
 
 
@@ -104,6 +104,10 @@ def lib():
                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                   ctypes.c_void_p, ctypes.c_void_p]
         L.lfmm_tree3d.restype = ctypes.c_int
+        L.lfmm_fmm2d.argtypes = L.lfmm_fmm3d.argtypes
+        L.lfmm_fmm2d.restype = ctypes.c_int
+        L.lfmm_tree2d.argtypes = L.lfmm_tree3d.argtypes
+        L.lfmm_tree2d.restype = ctypes.c_int
         L.lfmm_version.restype = ctypes.c_char_p
         L.lfmm_host_wigner_d.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
         L.lfmm_host_wigner_d.restype = ctypes.c_int
@@ -162,18 +166,11 @@ def _is_torch_cuda(x):
     return hasattr(x, "is_cuda") and x.is_cuda
 
 
-def fmm3d(xyz, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exclude_self=False,
-          timings=False, count_ops=False, ctx=None, out=None, targets=None, mixed=False):
-    """3D FMM: potential (m,) and field (m,3) at every target.
-
-    xyz (n,3) and q (n,) are the sources; targets (m,3) defaults to the
-    sources themselves (m = n).  float64 numpy arrays take the host path,
-    CUDA torch tensors the device path (outputs are then CUDA tensors, or
-    written into `out=(pot, field)`).  Returns (pot, field, report_dict).
-    mixed=True runs the M2L of levels >= 4 on the matrix cores in split FP16
-    with FP32 accumulation (LFMM_FLAG_MIXED); everything else stays FP64.
-    """
+def _solve(dim, xyz, q, eps, terms, tree_depth, leaf_max, exclude_self, timings, count_ops, ctx, out, targets,
+           mixed):
     ctx = ctx or default_context()
+    fn = lib().lfmm_fmm3d if dim == 3 else lib().lfmm_fmm2d
+    name = "lfmm_fmm3d" if dim == 3 else "lfmm_fmm2d"
     flags = ((FLAG_EXCLUDE_SELF if exclude_self else 0) | (FLAG_TIMINGS if timings else 0) |
              (FLAG_COUNT_OPS if count_ops else 0) | (FLAG_MIXED if mixed else 0))
     rep = Report()
@@ -194,56 +191,76 @@ def fmm3d(xyz, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exc
             tg = ctypes.byref(Points3d(m, targets.data_ptr(), targets.stride(0), None, 1))
         if out is None:
             pot = torch.empty(m, dtype=torch.float64, device=xyz.device)
-            field = torch.empty((m, 3), dtype=torch.float64, device=xyz.device)
+            field = torch.empty((m, dim), dtype=torch.float64, device=xyz.device)
         else:
             pot, field = out
-        rc = lib().lfmm_fmm3d(ctx.handle, ctypes.byref(pts), tg, float(eps), int(terms), int(tree_depth),
-                              int(leaf_max), MEM_DEVICE, flags, ctypes.c_void_p(pot.data_ptr()),
-                              ctypes.c_void_p(field.data_ptr()), ctypes.byref(rep))
-        _check(rc, "lfmm_fmm3d")
+        rc = fn(ctx.handle, ctypes.byref(pts), tg, float(eps), int(terms), int(tree_depth), int(leaf_max), MEM_DEVICE,
+                flags, ctypes.c_void_p(pot.data_ptr()), ctypes.c_void_p(field.data_ptr()), ctypes.byref(rep))
+        _check(rc, name)
         return pot, field, rep.as_dict()
-    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, dim)
     q = np.ascontiguousarray(q, dtype=np.float64)
     n = len(q)
-    pts = Points3d(n, xyz.ctypes.data, 3, q.ctypes.data, 1)
+    pts = Points3d(n, xyz.ctypes.data, dim, q.ctypes.data, 1)
     tg = None
     m = n
     if targets is not None:
-        targets = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+        targets = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, dim)
         m = len(targets)
-        tg = ctypes.byref(Points3d(m, targets.ctypes.data, 3, None, 1))
+        tg = ctypes.byref(Points3d(m, targets.ctypes.data, dim, None, 1))
     pot = np.empty(m)
-    field = np.empty((m, 3))
-    rc = lib().lfmm_fmm3d(ctx.handle, ctypes.byref(pts), tg, float(eps), int(terms), int(tree_depth),
-                          int(leaf_max), MEM_HOST, flags, pot.ctypes.data, field.ctypes.data, ctypes.byref(rep))
-    _check(rc, "lfmm_fmm3d")
+    field = np.empty((m, dim))
+    rc = fn(ctx.handle, ctypes.byref(pts), tg, float(eps), int(terms), int(tree_depth), int(leaf_max), MEM_HOST, flags,
+            pot.ctypes.data, field.ctypes.data, ctypes.byref(rep))
+    _check(rc, name)
     return pot, field, rep.as_dict()
 
 
-def tree3d(xyz, q, tree_depth, leaf_max=0, ctx=None):
-    """The GPU Morton tree: dict with depth, level sizes, geometry (shift, scale),
-    pos_map (input -> sorted position), per-level keys and ranges.  With
-    charges q, zero-charge particles follow the others inside each leaf."""
+def fmm3d(xyz, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exclude_self=False,
+          timings=False, count_ops=False, ctx=None, out=None, targets=None, mixed=False):
+    """3D FMM: potential (m,) and field (m,3) at every target.
+
+    xyz (n,3) and q (n,) are the sources; targets (m,3) defaults to the
+    sources themselves (m = n).  float64 numpy arrays take the host path,
+    CUDA torch tensors the device path (outputs are then CUDA tensors, or
+    written into `out=(pot, field)`).  Returns (pot, field, report_dict).
+    mixed=True runs the M2L of levels >= 4 on the matrix cores in split FP16
+    with FP32 accumulation (LFMM_FLAG_MIXED); everything else stays FP64.
+    """
+    return _solve(3, xyz, q, eps, terms, tree_depth, leaf_max, exclude_self, timings, count_ops, ctx, out, targets,
+                  mixed)
+
+
+def fmm2d(xy, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exclude_self=False,
+          timings=False, count_ops=False, ctx=None, out=None, targets=None):
+    """2D FMM (complex series): potential (m,) = sum q log|z - z_j| and the
+    complex force (m,2) = (re, im) of sum q / (z - z_j) at every target.
+    Arguments as fmm3d with 2 coordinates per point."""
+    return _solve(2, xy, q, eps, terms, tree_depth, leaf_max, exclude_self, timings, count_ops, ctx, out, targets,
+                  False)
+
+
+def _tree(dim, xyz, q, tree_depth, leaf_max, ctx):
     ctx = ctx or default_context()
-    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, dim)
     n = len(xyz)
     if q is not None:
         q = np.ascontiguousarray(q, dtype=np.float64)
-    pts = Points3d(n, xyz.ctypes.data, 3, q.ctypes.data if q is not None else None, 1)
+    pts = Points3d(n, xyz.ctypes.data, dim, q.ctypes.data if q is not None else None, 1)
     sizes = np.zeros(32, dtype=np.int64)
     geom = np.zeros(4)
     pos_map = np.zeros(n, dtype=np.int64)
     depth = np.zeros(1, dtype=np.int32)
-    # first call sizes the key/range buffers: total boxes <= n * depth
-    cap = n * (max(int(tree_depth), 1) if tree_depth != FMM_AUTO else 17) + 64
+    # total boxes <= n * depth
+    cap = n * (max(int(tree_depth), 1) if tree_depth != FMM_AUTO else 32) + 64
     keys = np.zeros(cap, dtype=np.uint64)
     ranges = np.zeros(cap + 64, dtype=np.int64)
-    rc = lib().lfmm_tree3d(ctx.handle, ctypes.byref(pts), int(tree_depth), int(leaf_max), depth.ctypes.data,
-                           sizes.ctypes.data, geom.ctypes.data, pos_map.ctypes.data, keys.ctypes.data,
-                           ranges.ctypes.data)
-    _check(rc, "lfmm_tree3d")
+    fn = lib().lfmm_tree3d if dim == 3 else lib().lfmm_tree2d
+    rc = fn(ctx.handle, ctypes.byref(pts), int(tree_depth), int(leaf_max), depth.ctypes.data, sizes.ctypes.data,
+            geom.ctypes.data, pos_map.ctypes.data, keys.ctypes.data, ranges.ctypes.data)
+    _check(rc, "lfmm_tree3d" if dim == 3 else "lfmm_tree2d")
     d = int(depth[0])
-    out = {"depth": d, "level_sizes": sizes[:d].copy(), "shift": geom[:3].copy(), "scale": float(geom[3]),
+    out = {"depth": d, "level_sizes": sizes[:d].copy(), "shift": geom[:dim].copy(), "scale": float(geom[3]),
            "pos_map": pos_map, "keys": [], "begin": []}
     o = b = 0
     for l in range(d):
@@ -253,6 +270,18 @@ def tree3d(xyz, q, tree_depth, leaf_max=0, ctx=None):
         o += s
         b += s + 1
     return out
+
+
+def tree3d(xyz, q, tree_depth, leaf_max=0, ctx=None):
+    """The GPU Morton octree: dict with depth, level sizes, geometry (shift,
+    scale), pos_map (input -> sorted position), per-level keys and ranges.
+    With charges q, zero-charge particles follow the others inside each leaf."""
+    return _tree(3, xyz, q, tree_depth, leaf_max, ctx)
+
+
+def tree2d(xy, q, tree_depth, leaf_max=0, ctx=None):
+    """The GPU Morton quadtree (keys x << 1 | y), as tree3d."""
+    return _tree(2, xy, q, tree_depth, leaf_max, ctx)
 
 
 def host_wigner_d(j, beta):

@@ -7,10 +7,7 @@
 //   -> M2L+L2L (down) -> P2P+L2P (leaves, scattered to input order).
 // Everything runs on one HIP stream; the only host synchronisations are the
 // box counts of each level (needed to size the next launch).
-#include "lfmm.h"
-#include "lfmm_internal.h"
-#include "ops3d.h"
-#include "tree.h"
+#include "api_internal.h"
 
 #include <hip/hip_runtime.h>
 
@@ -24,106 +21,12 @@
 #include <new>
 #include <vector>
 
-namespace {
+namespace lfmm::host {
 
 using namespace lfmm;
 
-constexpr int kMaxDepth = 16;   // 15 bits per axis at the leaves
-constexpr int kLutMaxLevel = 8; // dense key -> box tables up to 8^8 entries
-
-struct Arena {
-    struct Chunk { char* p; size_t size; };
-    std::vector<Chunk> chunks;
-    size_t cur = 0, off = 0, used = 0, peak = 0;
-
-    void reset()
-    {
-        if (chunks.size() > 1) {  // consolidate into one chunk sized for the last peak
-            for (auto& c : chunks) (void)hipFree(c.p);
-            chunks.clear();
-            const size_t want = peak + peak / 8 + (1 << 20);
-            char* p = nullptr;
-            if (hipMalloc(&p, want) == hipSuccess) chunks.push_back({p, want});
-        }
-        cur = 0;
-        off = 0;
-        used = 0;
-    }
-    void* alloc(size_t bytes)
-    {
-        bytes = (bytes + 255) & ~(size_t)255;
-        if (bytes == 0) bytes = 256;
-        while (cur < chunks.size()) {
-            if (off + bytes <= chunks[cur].size) {
-                void* p = chunks[cur].p + off;
-                off += bytes;
-                used += bytes;
-                peak = std::max(peak, used);
-                return p;
-            }
-            ++cur;
-            off = 0;
-        }
-        size_t sz = std::max<size_t>(bytes, (size_t)64 << 20);
-        if (!chunks.empty()) sz = std::max(sz, chunks.back().size);
-        char* p = nullptr;
-        if (hipMalloc(&p, sz) != hipSuccess) return nullptr;
-        chunks.push_back({p, sz});
-        cur = chunks.size() - 1;
-        off = bytes;
-        used += bytes;
-        peak = std::max(peak, used);
-        return p;
-    }
-    void release()
-    {
-        for (auto& c : chunks) (void)hipFree(c.p);
-        chunks.clear();
-        cur = off = used = peak = 0;
-    }
-};
-
-struct DevTables {
-    int N = 0;
-    void* block = nullptr;
-    RtrTables rt{};
-    const double* knm = nullptr;
-    uint4* m16 = nullptr;      // split-fp16 dense M2L operators (built on first mixed solve)
-    double m16_kinv = 0.0;
-};
-
 // First M2L level on the matrix cores: the two coarsest levels stay FP64.
 constexpr int kMixedFirstLevel = 4;
-
-struct Status {
-    int code = LFMM_OK;
-};
-
-}  // namespace
-
-struct lfmm_ctx {
-    int device = 0;
-    hipStream_t own_stream = nullptr;
-    hipStream_t stream = nullptr;
-    Arena arena;
-    std::vector<hipEvent_t> events;  // timing events, created on first timed call
-    std::map<int, DevTables> tables;
-    std::mutex mu;
-};
-
-namespace {
-
-#define LFMM_HIP(call)                                   \
-    do {                                                 \
-        hipError_t e_ = (call);                          \
-        if (e_ != hipSuccess) return LFMM_ERR_HIP;       \
-    } while (0)
-
-template <typename T>
-T* take(Arena& a, int64_t count)
-{
-    return static_cast<T*>(a.alloc(sizeof(T) * (size_t)std::max<int64_t>(count, 1)));
-}
 
 int get_tables(lfmm_ctx* ctx, int N, const DevTables** out)
 {
@@ -219,36 +122,8 @@ double ms_since(std::chrono::steady_clock::time_point t0)
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-// Device-resident Morton tree of one solve (all buffers in the context arena).
-struct Tree {
-    int64_t n = 0;
-    int depth = 0;
-    const double* dpos = nullptr;  // device view of the inputs
-    const double* dq = nullptr;
-    int64_t ps = 0, qs = 0;
-    double* geom = nullptr;        // {shift x, y, z, scale}
-    uint32_t* umax = nullptr;
-    uint32_t* order = nullptr;     // sorted -> input index
-    double4* src = nullptr;        // sorted (x, y, z, q)
-    int32_t* leaf_of = nullptr;
-    std::vector<int64_t> cnt;
-    std::vector<uint64_t*> lkeys;
-    std::vector<int32_t*> lbegin;  // leaves: particle ranges; above: child ranges
-    std::vector<int32_t*> lut;
-    uint32_t max_leaf = 0;
-};
-
-// Device view of the particle set a solve works on.
-struct DevPoints {
-    const double* pos = nullptr;
-    int64_t ps = 3;
-    const double* q = nullptr;
-    int64_t qs = 1;
-    int64_t n = 0;
-};
-
 // Copies a host view to the device (keeping its strides).
-int upload_view(lfmm_ctx* ctx, const lfmm_points3d* S, DevPoints& P)
+int upload_view(lfmm_ctx* ctx, const lfmm_points3d* S, int dim, DevPoints& P)
 {
     const int64_t n = S->n;
     const int64_t ps = S->pos_stride, qs = S->q_stride;
@@ -257,8 +132,9 @@ int upload_view(lfmm_ctx* ctx, const lfmm_points3d* S, DevPoints& P)
     P.n = n;
     P.ps = ps;
     P.qs = qs;
-    const size_t pbytes = sizeof(double) * (size_t)((n - 1) * ps + 3);
-    const bool shared = (S->q > S->pos && S->q < S->pos + (n - 1) * ps + 3 && ps == qs);
+    P.dim = dim;
+    const size_t pbytes = sizeof(double) * (size_t)((n - 1) * ps + dim);
+    const bool shared = (S->q > S->pos && S->q < S->pos + (n - 1) * ps + dim && ps == qs);
     if (shared) {  // one interleaved block, e.g. the reference's particle AoS
         const size_t span = sizeof(double) * (size_t)((n - 1) * ps + (S->q - S->pos) + 1);
         const size_t bytes = std::max(span, pbytes);
@@ -286,19 +162,19 @@ int upload_view(lfmm_ctx* ctx, const lfmm_points3d* S, DevPoints& P)
 // The particle set of a solve: the sources alone, or [targets (q = 0);
 // sources] as the reference's source/target constructor builds it
 // (src/cpu/multipole3d.cpp:1311-1319), packed as (x, y, z, q) records.
-int stage_inputs(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, int memory, DevPoints& P)
+int stage_inputs(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, int memory, int dim, DevPoints& P)
 {
     DevPoints s, t;
     if (memory == LFMM_MEM_HOST) {
-        int rc = upload_view(ctx, S, s);
+        int rc = upload_view(ctx, S, dim, s);
         if (rc != LFMM_OK) return rc;
         if (Tg) {
-            rc = upload_view(ctx, Tg, t);
+            rc = upload_view(ctx, Tg, dim, t);
             if (rc != LFMM_OK) return rc;
         }
     } else {
-        s = DevPoints{S->pos, S->pos_stride, S->q, S->q_stride, S->n};
-        if (Tg) t = DevPoints{Tg->pos, Tg->pos_stride, nullptr, 1, Tg->n};
+        s = DevPoints{S->pos, S->pos_stride, S->q, S->q_stride, S->n, dim};
+        if (Tg) t = DevPoints{Tg->pos, Tg->pos_stride, nullptr, 1, Tg->n, dim};
     }
     if (!Tg) {
         P = s;
@@ -307,9 +183,9 @@ int stage_inputs(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg,
     const int64_t n = t.n + s.n;
     double4* buf = take<double4>(ctx->arena, n);
     if (!buf) return LFMM_ERR_NOMEM;
-    LFMM_HIP(pack_points(t.pos, t.ps, nullptr, 1, t.n, buf, ctx->stream));
-    LFMM_HIP(pack_points(s.pos, s.ps, s.q, s.qs, s.n, buf + t.n, ctx->stream));
-    P = DevPoints{(const double*)buf, 4, (const double*)buf + 3, 4, n};
+    LFMM_HIP(pack_points(dim, t.pos, t.ps, nullptr, 1, t.n, buf, ctx->stream));
+    LFMM_HIP(pack_points(dim, s.pos, s.ps, s.q, s.qs, s.n, buf + t.n, ctx->stream));
+    P = DevPoints{(const double*)buf, 4, (const double*)buf + 3, 4, n, dim};
     return LFMM_OK;
 }
 
@@ -319,9 +195,11 @@ int stage_inputs(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg,
 int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, bool partition, Tree& T)
 {
     const int64_t n = DP.n;
+    const int dim = DP.dim;
     hipStream_t st = ctx->stream;
     Arena& A = ctx->arena;
     T.n = n;
+    T.dim = dim;
     const double* dpos = DP.pos;
     const double* dq = DP.q;
     const int64_t ps = DP.ps, qs = DP.qs;
@@ -338,7 +216,7 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
     T.geom = geom;
     T.umax = umax;
     LFMM_HIP(hipMemsetAsync(nonfinite, 0, sizeof(int), st));
-    LFMM_HIP(extent(dpos, ps, n, part, nonfinite, geom, st));
+    LFMM_HIP(extent(dim, dpos, ps, n, part, nonfinite, geom, st));
     {
         int bad = 0;
         LFMM_HIP(hipMemcpyAsync(&bad, nonfinite, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -351,19 +229,21 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
     if (depth_in == LFMM_AUTO) {
         if (leaf_max > 0) {
             int maxl = 1;
-            while (maxl < kLutMaxLevel && ((int64_t)1 << (3 * maxl)) < n) ++maxl;
+            while (maxl < lut_max_level(dim) && ((int64_t)1 << (dim * maxl)) < n) ++maxl;
             int64_t cnt = 0;
-            for (int l = 0; l <= maxl; ++l) cnt += (int64_t)1 << (3 * l);
+            for (int l = 0; l <= maxl; ++l) cnt += (int64_t)1 << (dim * l);
             uint32_t* cws = take<uint32_t>(A, cnt);
             if (!cws) return LFMM_ERR_NOMEM;
             int lvl = 0;
-            LFMM_HIP(occupancy_depth(dpos, ps, n, geom, leaf_max, maxl, cws, umax, &lvl, st));
+            LFMM_HIP(occupancy_depth(dim, dpos, ps, n, geom, leaf_max, maxl, cws, umax, &lvl, st));
             depth = lvl + 1;
-        } else {
+        } else if (dim == 3) {  // src/cpu/multipole3d.cpp:1334-1337
             depth = (int)std::max(2.0, 1 + std::log((double)n / 100.0) / std::log(8.0));
+        } else {                // src/cpu/multipole.cpp:483-487
+            depth = (int)std::max(3.0, 2 + std::log((double)n / 100.0) / std::log(4.0));
         }
     }
-    if (depth < 1 || depth > kMaxDepth) return LFMM_ERR_DEPTH;
+    if (depth < 1 || depth > (dim == 3 ? kMaxDepth : 2 * kMaxDepth - 1)) return LFMM_ERR_DEPTH;
     T.depth = depth;
     const int leaf_level = depth - 1;
 
@@ -372,10 +252,10 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
     uint32_t* v0 = take<uint32_t>(A, n);
     void* sws = A.alloc((size_t)sort_ws_bytes(n));
     if (!k0 || !v0 || !sws) return LFMM_ERR_NOMEM;
-    LFMM_HIP(keys(dpos, ps, n, leaf_level, geom, k0, v0, st));
+    LFMM_HIP(keys(dim, dpos, ps, n, leaf_level, geom, k0, v0, st));
     uint64_t* skeys = k0;
     uint32_t* order = v0;
-    if (leaf_level > 0) LFMM_HIP(radix_sort(k0, v0, n, 3 * leaf_level, sws, st, &skeys, &order));
+    if (leaf_level > 0) LFMM_HIP(radix_sort(k0, v0, n, dim * leaf_level, sws, st, &skeys, &order));
     T.order = order;
 
     // ---- levels
@@ -389,7 +269,7 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
     T.leaf_of = take<int32_t>(A, n);
     if (!flags_ws || !excl_ws || !scan_ws || !T.leaf_of) return LFMM_ERR_NOMEM;
     {
-        const int64_t cap = std::min<int64_t>(n, (int64_t)1 << std::min(60, 3 * leaf_level));
+        const int64_t cap = std::min<int64_t>(n, (int64_t)1 << std::min(60, dim * leaf_level));
         T.lkeys[leaf_level] = take<uint64_t>(A, cap);
         T.lbegin[leaf_level] = take<int32_t>(A, cap + 1);
         if (!T.lkeys[leaf_level] || !T.lbegin[leaf_level]) return LFMM_ERR_NOMEM;
@@ -401,22 +281,22 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
         if (partition) LFMM_HIP(partition_leaves(T.lbegin[leaf_level], T.cnt[leaf_level], order, dq, qs, st));
         T.src = take<double4>(A, n);
         if (!T.src) return LFMM_ERR_NOMEM;
-        LFMM_HIP(gather_sources(dpos, ps, dq, qs, order, n, T.src, st));
+        LFMM_HIP(gather_sources(dim, dpos, ps, dq, qs, order, n, T.src, st));
     }
     for (int l = leaf_level - 1; l >= 0; --l) {
         const int64_t m = T.cnt[l + 1];
-        const int64_t cap = std::min<int64_t>(m, (int64_t)1 << (3 * l));
+        const int64_t cap = std::min<int64_t>(m, (int64_t)1 << (dim * l));
         T.lkeys[l] = take<uint64_t>(A, cap);
         T.lbegin[l] = take<int32_t>(A, cap + 1);
         if (!T.lkeys[l] || !T.lbegin[l]) return LFMM_ERR_NOMEM;
-        LFMM_HIP(run_compact(T.lkeys[l + 1], m, 3, flags_ws, excl_ws, scan_ws, T.lkeys[l], T.lbegin[l], nullptr,
+        LFMM_HIP(run_compact(T.lkeys[l + 1], m, dim, flags_ws, excl_ws, scan_ws, T.lkeys[l], T.lbegin[l], nullptr,
                              &T.cnt[l], st));
     }
     for (int l = 2; l < depth; ++l) {
-        if (l > kLutMaxLevel) continue;
-        T.lut[l] = take<int32_t>(A, (int64_t)1 << (3 * l));
+        if (l > lut_max_level(dim)) continue;
+        T.lut[l] = take<int32_t>(A, (int64_t)1 << (dim * l));
         if (!T.lut[l]) return LFMM_ERR_NOMEM;
-        LFMM_HIP(build_lut(T.lkeys[l], T.cnt[l], l, T.lut[l], st));
+        LFMM_HIP(build_lut(dim, T.lkeys[l], T.cnt[l], l, T.lut[l], st));
     }
     LFMM_HIP(max_run(T.lbegin[leaf_level], T.cnt[leaf_level], umax, &T.max_leaf, st));
     return LFMM_OK;
@@ -436,7 +316,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     Arena& A = ctx->arena;
     A.reset();
     DevPoints DP;
-    int rc = stage_inputs(ctx, S, Tg, memory, DP);
+    int rc = stage_inputs(ctx, S, Tg, memory, 3, DP);
     if (rc != LFMM_OK) return rc;
     const int64_t n = DP.n;
     Tree T;
@@ -503,6 +383,9 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         if (!xp || !xmax) return LFMM_ERR_NOMEM;
         LFMM_HIP(fill_u32(xmax, depth, 0u, st));
     }
+    // direction parts: levels with fewer workgroups than the chip holds split
+    // their 189 directions over more workgroups (deterministic partial sums)
+    auto dsplit_for = [](int64_t wgs) { return (int)std::max<int64_t>(1, std::min<int64_t>(32, (1024 + wgs - 1) / wgs)); };
     if (far) {
         for (int l = 2; l < depth; ++l) {
             RtrLevel L{};
@@ -516,6 +399,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             L.src = outer[l];
             L.parent_local = (l > 2) ? inner[l - 1] : nullptr;
             L.dst = inner[l];
+            L.tiles = (int)((cnt[l - 1] + rtr_tile(N) - 1) / rtr_tile(N));
             if (mixed && l >= kMixedFirstLevel) {
                 // FP64 L2L into the level, then += the matrix-core M2L
                 M2L16Args m{};
@@ -534,17 +418,31 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                 m.size = L.size;
                 m.kinv = DT->m16_kinv;
                 m.dst = inner[l];
+                m.tiles = (int)((cnt[l - 1] + m16_targets(N) - 1) / m16_targets(N));
+                m.dsplit = dsplit_for((int64_t)m.tiles * 8);
+                if (m.dsplit > 1) {
+                    m.partial = take<double2>(A, (int64_t)m.dsplit * cnt[l] * NX);
+                    if (!m.partial) return LFMM_ERR_NOMEM;
+                    LFMM_HIP(fill_u32((uint32_t*)m.partial, (int64_t)m.dsplit * cnt[l] * NX * 4, 0u, st));
+                }
                 LFMM_HIP(launch_absmax((const double*)outer[l], cnt[l] * NX * 2, xmax + l, st));
                 LFMM_HIP(launch_m2l16_pack(N, m, st));
                 LFMM_HIP(launch_rtr(N, 2, DT->rt, L, st));
                 if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
                 LFMM_HIP(launch_m2l16(N, m, st));
+                if (m.dsplit > 1) LFMM_HIP(launch_sum_parts(m.partial, m.dsplit, cnt[l] * NX, inner[l], true, st));
                 if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
                 ev_mixed.push_back(1);
                 ++mixed_launches;
             } else {
+                L.dsplit = dsplit_for((int64_t)L.tiles * 8);
+                if (L.dsplit > 1) {
+                    L.partial = take<double2>(A, (int64_t)L.dsplit * cnt[l] * NX);
+                    if (!L.partial) return LFMM_ERR_NOMEM;
+                }
                 if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
                 LFMM_HIP(launch_rtr(N, 1, DT->rt, L, st));
+                if (L.dsplit > 1) LFMM_HIP(launch_sum_parts(L.partial, L.dsplit, cnt[l] * NX, inner[l], false, st));
                 if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
                 ev_mixed.push_back(0);
             }
@@ -573,7 +471,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     la.nleaf = (int)cnt[leaf_level];
     la.level = leaf_level;
     la.geom = geom;
-    la.lut = leaf_level <= kLutMaxLevel ? (leaf_level >= 2 ? lut[leaf_level] : nullptr) : nullptr;
+    la.lut = leaf_level <= lut_max_level(3) ? (leaf_level >= 2 ? lut[leaf_level] : nullptr) : nullptr;
     la.local = far ? inner[leaf_level] : nullptr;
     la.knm = DT->knm;
     la.perm = order;
@@ -625,7 +523,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             unsigned long long* cws = take<unsigned long long>(A, 2 + depth);
             if (!cws) return LFMM_ERR_NOMEM;
             std::vector<int64_t> per(depth, 0);
-            LFMM_HIP(count_ops(lkeys, lbegin, lut, cnt, cws, &rep->m2l_ops, &rep->p2p_pairs, per.data(), st));
+            LFMM_HIP(count_ops(3, lkeys, lbegin, lut, cnt, cws, &rep->m2l_ops, &rep->p2p_pairs, per.data(), st));
             rep->m2l_ops_mixed = 0;
             if (mixed)
                 for (int l = kMixedFirstLevel; l < depth; ++l) rep->m2l_ops_mixed += per[l];
@@ -634,16 +532,69 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     return LFMM_OK;
 }
 
-bool valid_points(const lfmm_points3d* p, bool need_q)
+bool valid_points(const lfmm_points3d* p, bool need_q, int dim)
 {
     if (!p || p->n < 0) return false;
     if (p->n == 0) return true;
-    if (!p->pos || p->pos_stride < 3) return false;
+    if (!p->pos || p->pos_stride < dim) return false;
     if (need_q && (!p->q || p->q_stride < 1)) return false;
     return p->n < ((int64_t)1 << 31);
 }
 
-}  // namespace
+static int tree_export(lfmm_ctx* c, const lfmm_points3d* particles, int dim, int tree_depth, int leaf_max,
+                       int32_t* depth_out, int64_t* level_sizes, double* geom, int64_t* pos_map, uint64_t* keys,
+                       int64_t* begins)
+{
+    if (!c || !valid_points(particles, false, dim) || particles->n == 0) return LFMM_ERR_INVALID;
+    if (tree_depth != LFMM_AUTO && (tree_depth < 1 || tree_depth > (dim == 3 ? kMaxDepth : 2 * kMaxDepth - 1)))
+        return LFMM_ERR_DEPTH;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return LFMM_ERR_HIP;
+    try {
+        c->arena.reset();
+        Tree T;
+        DevPoints DP;
+        int rc = upload_view(c, particles, dim, DP);
+        if (rc != LFMM_OK) return rc;
+        rc = build_tree(c, DP, tree_depth, leaf_max, true, T);
+        if (rc != LFMM_OK) return rc;
+        hipStream_t st = c->stream;
+        const int64_t n = T.n;
+        if (depth_out) *depth_out = T.depth;
+        if (level_sizes)
+            for (int l = 0; l < T.depth; ++l) level_sizes[l] = T.cnt[l];
+        if (geom) LFMM_HIP(hipMemcpyAsync(geom, T.geom, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (pos_map) {
+            std::vector<uint32_t> ord(n);
+            LFMM_HIP(hipMemcpyAsync(ord.data(), T.order, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            LFMM_HIP(hipStreamSynchronize(st));
+            for (int64_t s = 0; s < n; ++s) pos_map[ord[s]] = s;
+        }
+        int64_t ko = 0, bo = 0;
+        for (int l = 0; l < T.depth; ++l) {
+            if (keys) LFMM_HIP(hipMemcpyAsync(keys + ko, T.lkeys[l], T.cnt[l] * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+            if (begins) {
+                std::vector<int32_t> b(T.cnt[l] + 1);
+                LFMM_HIP(hipMemcpyAsync(b.data(), T.lbegin[l], (T.cnt[l] + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+                LFMM_HIP(hipStreamSynchronize(st));
+                for (int64_t i = 0; i <= T.cnt[l]; ++i) begins[bo + i] = b[i];
+            }
+            ko += T.cnt[l];
+            bo += T.cnt[l] + 1;
+        }
+        LFMM_HIP(hipStreamSynchronize(st));
+        return LFMM_OK;
+    } catch (const std::bad_alloc&) {
+        return LFMM_ERR_NOMEM;
+    } catch (...) {
+        return LFMM_ERR_INVALID;
+    }
+}
+
+}  // namespace lfmm::host
+
+using namespace lfmm;
+using namespace lfmm::host;
 
 extern "C" {
 
@@ -677,6 +628,7 @@ void lfmm_destroy(lfmm_ctx* c)
         (void)hipFree(kv.second.block);
         (void)hipFree(kv.second.m16);
     }
+    free_tables2d(c);
     for (auto e : c->events) (void)hipEventDestroy(e);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
@@ -719,12 +671,12 @@ int lfmm_fmm3d(lfmm_ctx* c, const lfmm_points3d* sources, const lfmm_points3d* t
                int tree_depth, int leaf_max, int memory, unsigned flags, double* potential, double* field,
                lfmm_report* report)
 {
-    if (!c || !valid_points(sources, true)) return LFMM_ERR_INVALID;
+    if (!c || !valid_points(sources, true, 3)) return LFMM_ERR_INVALID;
     if (memory != LFMM_MEM_HOST && memory != LFMM_MEM_DEVICE) return LFMM_ERR_INVALID;
     if (terms != LFMM_AUTO && terms < 1) return LFMM_ERR_TERMS;
     if (tree_depth != LFMM_AUTO && (tree_depth < 1 || tree_depth > kMaxDepth)) return LFMM_ERR_DEPTH;
     if (!(eps > 0.0) && terms == LFMM_AUTO) return LFMM_ERR_INVALID;
-    if (targets && !valid_points(targets, false)) return LFMM_ERR_INVALID;
+    if (targets && !valid_points(targets, false, 3)) return LFMM_ERR_INVALID;
     const lfmm_points3d* tg = (targets && targets->n > 0) ? targets : nullptr;
     if (tg && sources->n + tg->n >= ((int64_t)1 << 31)) return LFMM_ERR_INVALID;
     if (report) std::memset(report, 0, sizeof(*report));
@@ -756,49 +708,13 @@ int lfmm_fmm3d(lfmm_ctx* c, const lfmm_points3d* sources, const lfmm_points3d* t
 int lfmm_tree3d(lfmm_ctx* c, const lfmm_points3d* particles, int tree_depth, int leaf_max, int32_t* depth_out,
                 int64_t* level_sizes, double* geom, int64_t* pos_map, uint64_t* keys, int64_t* begins)
 {
-    if (!c || !valid_points(particles, false) || particles->n == 0) return LFMM_ERR_INVALID;
-    if (tree_depth != LFMM_AUTO && (tree_depth < 1 || tree_depth > kMaxDepth)) return LFMM_ERR_DEPTH;
-    std::lock_guard<std::mutex> g(c->mu);
-    if (hipSetDevice(c->device) != hipSuccess) return LFMM_ERR_HIP;
-    try {
-        c->arena.reset();
-        Tree T;
-        DevPoints DP;
-        int rc = upload_view(c, particles, DP);
-        if (rc != LFMM_OK) return rc;
-        rc = build_tree(c, DP, tree_depth, leaf_max, true, T);
-        if (rc != LFMM_OK) return rc;
-        hipStream_t st = c->stream;
-        const int64_t n = T.n;
-        if (depth_out) *depth_out = T.depth;
-        if (level_sizes)
-            for (int l = 0; l < T.depth; ++l) level_sizes[l] = T.cnt[l];
-        if (geom) LFMM_HIP(hipMemcpyAsync(geom, T.geom, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
-        if (pos_map) {
-            std::vector<uint32_t> ord(n);
-            LFMM_HIP(hipMemcpyAsync(ord.data(), T.order, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-            LFMM_HIP(hipStreamSynchronize(st));
-            for (int64_t s = 0; s < n; ++s) pos_map[ord[s]] = s;
-        }
-        int64_t ko = 0, bo = 0;
-        for (int l = 0; l < T.depth; ++l) {
-            if (keys) LFMM_HIP(hipMemcpyAsync(keys + ko, T.lkeys[l], T.cnt[l] * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-            if (begins) {
-                std::vector<int32_t> b(T.cnt[l] + 1);
-                LFMM_HIP(hipMemcpyAsync(b.data(), T.lbegin[l], (T.cnt[l] + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-                LFMM_HIP(hipStreamSynchronize(st));
-                for (int64_t i = 0; i <= T.cnt[l]; ++i) begins[bo + i] = b[i];
-            }
-            ko += T.cnt[l];
-            bo += T.cnt[l] + 1;
-        }
-        LFMM_HIP(hipStreamSynchronize(st));
-        return LFMM_OK;
-    } catch (const std::bad_alloc&) {
-        return LFMM_ERR_NOMEM;
-    } catch (...) {
-        return LFMM_ERR_INVALID;
-    }
+    return tree_export(c, particles, 3, tree_depth, leaf_max, depth_out, level_sizes, geom, pos_map, keys, begins);
+}
+
+int lfmm_tree2d(lfmm_ctx* c, const lfmm_points2d* particles, int tree_depth, int leaf_max, int32_t* depth_out,
+                int64_t* level_sizes, double* geom, int64_t* pos_map, uint64_t* keys, int64_t* begins)
+{
+    return tree_export(c, particles, 2, tree_depth, leaf_max, depth_out, level_sizes, geom, pos_map, keys, begins);
 }
 
 int lfmm_host_wigner_d(int j, double beta, double* out)

@@ -1,0 +1,158 @@
+// api_internal.h -- host-side internals shared by the 3D and 2D solves:
+// context, workspace arena, device tables, particle views and the GPU tree.
+#pragma once
+#include "lfmm.h"
+#include "lfmm_internal.h"
+#include "ops3d.h"
+#include "tree.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace lfmm::host {
+
+constexpr int kMaxDepth = 16;   // 15 bits per axis at the leaves (3D)
+// dense key -> box tables up to 2^24 entries (levels <= 8 in 3D, <= 12 in 2D)
+constexpr int lut_max_level(int dim) { return 24 / dim; }
+
+struct Arena {
+    struct Chunk { char* p; size_t size; };
+    std::vector<Chunk> chunks;
+    size_t cur = 0, off = 0, used = 0, peak = 0;
+
+    void reset()
+    {
+        if (chunks.size() > 1) {  // consolidate into one chunk sized for the last peak
+            for (auto& c : chunks) (void)hipFree(c.p);
+            chunks.clear();
+            const size_t want = peak + peak / 8 + (1 << 20);
+            char* p = nullptr;
+            if (hipMalloc(&p, want) == hipSuccess) chunks.push_back({p, want});
+        }
+        cur = 0;
+        off = 0;
+        used = 0;
+    }
+    void* alloc(size_t bytes)
+    {
+        bytes = (bytes + 255) & ~(size_t)255;
+        if (bytes == 0) bytes = 256;
+        while (cur < chunks.size()) {
+            if (off + bytes <= chunks[cur].size) {
+                void* p = chunks[cur].p + off;
+                off += bytes;
+                used += bytes;
+                peak = std::max(peak, used);
+                return p;
+            }
+            ++cur;
+            off = 0;
+        }
+        size_t sz = std::max<size_t>(bytes, (size_t)64 << 20);
+        if (!chunks.empty()) sz = std::max(sz, chunks.back().size);
+        char* p = nullptr;
+        if (hipMalloc(&p, sz) != hipSuccess) return nullptr;
+        chunks.push_back({p, sz});
+        cur = chunks.size() - 1;
+        off = bytes;
+        used += bytes;
+        peak = std::max(peak, used);
+        return p;
+    }
+    void release()
+    {
+        for (auto& c : chunks) (void)hipFree(c.p);
+        chunks.clear();
+        cur = off = used = peak = 0;
+    }
+};
+
+struct DevTables {
+    int N = 0;
+    void* block = nullptr;
+    RtrTables rt{};
+    const double* knm = nullptr;
+    uint4* m16 = nullptr;      // split-fp16 dense M2L operators (built on first mixed solve)
+    double m16_kinv = 0.0;
+};
+
+
+
+struct DevTables2d;  // api2d.cpp
+
+}  // namespace lfmm::host
+
+struct lfmm_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    lfmm::host::Arena arena;
+    std::vector<hipEvent_t> events;  // timing events, created on first timed call
+    std::map<int, lfmm::host::DevTables> tables;
+    std::map<int, lfmm::host::DevTables2d*> tables2d;
+    std::mutex mu;
+};
+
+
+namespace lfmm::host {
+
+#define LFMM_HIP(call)                                   \
+    do {                                                 \
+        hipError_t e_ = (call);                          \
+        if (e_ != hipSuccess) return LFMM_ERR_HIP;       \
+    } while (0)
+
+template <typename T>
+T* take(Arena& a, int64_t count)
+{
+    return static_cast<T*>(a.alloc(sizeof(T) * (size_t)std::max<int64_t>(count, 1)));
+}
+
+// Device-resident Morton tree of one solve (all buffers in the context arena).
+struct Tree {
+    int64_t n = 0;
+    int dim = 3;
+    int depth = 0;
+    const double* dpos = nullptr;  // device view of the inputs
+    const double* dq = nullptr;
+    int64_t ps = 0, qs = 0;
+    double* geom = nullptr;        // {shift x, y, z, scale}
+    uint32_t* umax = nullptr;
+    uint32_t* order = nullptr;     // sorted -> input index
+    double4* src = nullptr;        // sorted (x, y, z, q)
+    int32_t* leaf_of = nullptr;
+    std::vector<int64_t> cnt;
+    std::vector<uint64_t*> lkeys;
+    std::vector<int32_t*> lbegin;  // leaves: particle ranges; above: child ranges
+    std::vector<int32_t*> lut;
+    uint32_t max_leaf = 0;
+};
+
+// Device view of the particle set a solve works on.
+struct DevPoints {
+    const double* pos = nullptr;
+    int64_t ps = 3;
+    const double* q = nullptr;
+    int64_t qs = 1;
+    int64_t n = 0;
+    int dim = 3;
+};
+
+
+hipEvent_t ctx_event(lfmm_ctx* ctx, size_t i);
+double ms_since(std::chrono::steady_clock::time_point t0);
+// Copies a host view (dim coordinates per point) to the device, keeping its strides.
+int upload_view(lfmm_ctx* ctx, const lfmm_points3d* S, int dim, DevPoints& P);
+// The particle set of a solve: the sources alone, or [targets (q = 0); sources].
+int stage_inputs(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, int memory, int dim, DevPoints& P);
+int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, bool partition, Tree& T);
+bool valid_points(const lfmm_points3d* p, bool need_q, int dim);
+void free_tables2d(lfmm_ctx* ctx);
+
+}  // namespace lfmm::host

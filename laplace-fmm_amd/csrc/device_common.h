@@ -37,6 +37,32 @@ __host__ __device__ inline uint64_t morton3(uint32_t ix, uint32_t iy, uint32_t i
     return (spread3(ix) << 2) | (spread3(iy) << 1) | spread3(iz);
 }
 
+// 2D interleave: x is the high bit of each pair (x << 1 | y), as in the
+// reference's MortonCode<2> (src/cpu/morton_tree.cpp:84-91). 31 bits per axis.
+__host__ __device__ inline uint64_t spread2(uint64_t v)
+{
+    v &= 0x7fffffffull;
+    v = (v | (v << 16)) & 0x0000ffff0000ffffull;
+    v = (v | (v << 8)) & 0x00ff00ff00ff00ffull;
+    v = (v | (v << 4)) & 0x0f0f0f0f0f0f0f0full;
+    v = (v | (v << 2)) & 0x3333333333333333ull;
+    v = (v | (v << 1)) & 0x5555555555555555ull;
+    return v;
+}
+
+__host__ __device__ inline uint32_t compact2(uint64_t v)
+{
+    v &= 0x5555555555555555ull;
+    v = (v ^ (v >> 1)) & 0x3333333333333333ull;
+    v = (v ^ (v >> 2)) & 0x0f0f0f0f0f0f0f0full;
+    v = (v ^ (v >> 4)) & 0x00ff00ff00ff00ffull;
+    v = (v ^ (v >> 8)) & 0x0000ffff0000ffffull;
+    v = (v ^ (v >> 16)) & 0x00000000ffffffffull;
+    return (uint32_t)v;
+}
+
+__host__ __device__ inline uint64_t morton2(uint32_t ix, uint32_t iy) { return (spread2(ix) << 1) | spread2(iy); }
+
 // Box centre exactly as the reference's PointFromMortonCode
 // (src/cpu/morton_tree.cpp:122-145): ((2i+1)/2^(l+1)) * scale + shift with
 // the product and the sum rounded separately.

@@ -50,6 +50,21 @@ struct OpTables {
 
 void build_op_tables(int N, OpTables& T);
 
+// Host-built FP64 operator tables of the 2D FMM for N terms (complex values
+// as (re, im) pairs; see tables2d.cpp for the normalisation).
+constexpr int kMaxTerms2d = 30;
+constexpr int kNumM2LDirs2d = 40;   // offsets in [-3,3]^2 minus [-1,1]^2
+constexpr int kDirsPerClass2d = 27; // M2L offsets reachable from one child class
+struct OpTables2d {
+    int N = 0;
+    std::vector<int> dir_vec;      // [40][2]
+    std::vector<int> class_dirs;   // [4][27]
+    std::vector<double> m2l;       // [40][N][N] complex: out l, in j
+    std::vector<double> m2m;       // [4][N][N] complex (child class c)
+    std::vector<double> l2l;       // [4][N][N] complex (child class c)
+};
+void build_op_tables_2d(int N, OpTables2d& T);
+
 // Dense real form of the M2L operator of direction `dir` (< kNumM2LDirs):
 // K[row * N^2 + col], rows and columns in m2l_real_index order.
 int m2l_real_index(int n, int m, int part);

@@ -16,6 +16,27 @@ __global__ void __launch_bounds__(256) k_absmax(const double* __restrict__ v, in
     if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint((float)m));
 }
 
+__global__ void __launch_bounds__(256) k_sum_parts(const double2* __restrict__ partial, int parts, int64_t count,
+                                                   double2* __restrict__ dst, int add)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 v = add ? dst[i] : make_double2(0.0, 0.0);
+        for (int p = 0; p < parts; ++p) {
+            const double2 x = partial[(int64_t)p * count + i];
+            v.x += x.x;
+            v.y += x.y;
+        }
+        dst[i] = v;
+    }
+}
+
+hipError_t launch_sum_parts(const double2* partial, int parts, int64_t count, double2* dst, bool add, hipStream_t st)
+{
+    const int64_t blocks = std::min<int64_t>((count + 255) / 256, 8192);
+    if (blocks > 0) hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)blocks), dim3(256), 0, st, partial, parts, count, dst, add ? 1 : 0);
+    return hipGetLastError();
+}
+
 hipError_t launch_absmax(const double* v, int64_t count, uint32_t* out, hipStream_t st)
 {
     const int64_t blocks = std::min<int64_t>((count + 255) / 256, 2048);

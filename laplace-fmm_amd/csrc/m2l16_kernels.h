@@ -100,7 +100,13 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int blk = xcd_local_block(blockIdx.x, gridDim.x);
-    const int tile = blk >> 3, c = blk & 7;
+    // class-major block order: the workgroups resident on one XCD share a child
+    // class and direction part, so they stream the same operators (L2 reuse)
+    const int per_class = a.tiles * a.dsplit;
+    const int c = blk / per_class;
+    const int part = (blk - c * per_class) / a.tiles;
+    const int tile = blk - c * per_class - part * a.tiles;
+    const int d_beg = part * kDirsPerClass / a.dsplit, d_end = (part + 1) * kDirsPerClass / a.dsplit;
     const int side = 1 << a.level;
 
     // target box of column `col`: the class-c child of parent tile * TC + col
@@ -125,8 +131,7 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
     const int lcol = tid / TPC, lq = tid % TPC;
     uint32_t lx = 0, ly = 0, lz = 0;
     const int ltgt = target_of(lcol, lx, ly, lz);
-    uint4 nb[PPT];
-    auto load_b = [&](int d) {
+    auto source_piece = [&](int d) -> const uint4* {
         const int dir = class_dirs[c * kDirsPerClass + d];
         int box = a.size;  // the zero box
         if (ltgt >= 0) {
@@ -137,18 +142,17 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
                 if (s >= 0) box = s;
             }
         }
-        const uint4* p = xp + (int64_t)box * PIECES;
-#pragma unroll
-        for (int i = 0; i < PPT; ++i)
-            if (PIECES % TPC == 0 || lq + TPC * i < PIECES) nb[i] = p[lq + TPC * i];
+        return xp + (int64_t)box * PIECES + lq;
     };
-    auto store_b = [&](int buf) {
-        const int ct = lcol >> 5, cl = lcol & 31;
-#pragma unroll
-        for (int i = 0; i < PPT; ++i) {
-            const int pc = lq + TPC * i;
-            if (PIECES % TPC == 0 || pc < PIECES) bs[buf][ct][pc >> 2][(pc >> 1) & 1][(pc & 1) * 32 + cl] = nb[i];
-        }
+    auto stage_slot = [&](int buf, int i) -> uint4* {
+        const int pc = lq + TPC * i;
+        return &bs[buf][lcol >> 5][pc >> 2][(pc >> 1) & 1][(pc & 1) * 32 + (lcol & 31)];
+    };
+    constexpr bool kFullPieces = (PIECES % TPC) == 0;
+    auto piece_ok = [&](int i) { return kFullPieces || lq + TPC * i < PIECES; };
+    // operator fragments of this wave's row tiles for direction step d
+    auto op_base = [&](int d) -> const uint4* {
+        return kmat + (int64_t)class_dirs[c * kDirsPerClass + d] * (RT * KC * 2 * 64) + lane;
     };
 
     m16_acc acc[RW][CT];
@@ -159,26 +163,48 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[r][ct][v] = 0.0f;
 
-    load_b(0);
-    store_b(0);
-    lds_barrier();
-    for (int d = 0; d < kDirsPerClass; ++d) {
-        const int buf = d & 1;
-        const int dir = class_dirs[c * kDirsPerClass + d];
-        const uint4* ka = kmat + (int64_t)dir * (RT * KC * 2 * 64) + lane;
-        uint4 ah[RW][KC], al[RW][KC];
+    uint4 ah[RW][KC], al[RW][KC];
+    {
+        const uint4* p = source_piece(d_beg);
+        uint4 t[PPT];
 #pragma unroll
-        for (int r = 0; r < RW; ++r) {
-            const int rt = w + 4 * r;
-            if (rt < RT) {
+        for (int i = 0; i < PPT; ++i) t[i] = piece_ok(i) ? p[TPC * i] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < PPT; ++i)
+            if (piece_ok(i)) *stage_slot(0, i) = t[i];
+        const uint4* ka = op_base(d_beg);
+#pragma unroll
+        for (int r = 0; r < RW; ++r)
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) {
+                const int rt = min(w + 4 * r, RT - 1);
+                ah[r][kc] = ka[((rt * KC + kc) * 2) * 64];
+                al[r][kc] = ka[((rt * KC + kc) * 2 + 1) * 64];
+            }
+    }
+    lds_barrier();
+    for (int d = d_beg; d < d_end; ++d) {
+        const int buf = (d - d_beg) & 1;
+        const int dn = min(d + 1, d_end - 1);  // the last step re-stages its own data (never read)
+        // next step's sources and operator fragments, in flight under the MFMAs
+        uint4 nb[PPT];
+        {
+            const uint4* p = source_piece(dn);
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) nb[i] = piece_ok(i) ? p[TPC * i] : make_uint4(0, 0, 0, 0);
+        }
+        uint4 nah[RW][KC], nal[RW][KC];
+        {
+            const uint4* ka = op_base(dn);
+#pragma unroll
+            for (int r = 0; r < RW; ++r)
 #pragma unroll
                 for (int kc = 0; kc < KC; ++kc) {
-                    ah[r][kc] = ka[((rt * KC + kc) * 2) * 64];
-                    al[r][kc] = ka[((rt * KC + kc) * 2 + 1) * 64];
+                    const int rt = min(w + 4 * r, RT - 1);
+                    nah[r][kc] = ka[((rt * KC + kc) * 2) * 64];
+                    nal[r][kc] = ka[((rt * KC + kc) * 2 + 1) * 64];
                 }
-            }
         }
-        if (d + 1 < kDirsPerClass) load_b(d + 1);  // next step's sources, in flight under the MFMAs
 #pragma unroll
         for (int kc = 0; kc < KC; ++kc) {
 #pragma unroll
@@ -197,18 +223,29 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
                 }
             }
         }
-        if (d + 1 < kDirsPerClass) store_b(buf ^ 1);
+#pragma unroll
+        for (int i = 0; i < PPT; ++i)
+            if (piece_ok(i)) *stage_slot(buf ^ 1, i) = nb[i];
+#pragma unroll
+        for (int r = 0; r < RW; ++r)
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) {
+                ah[r][kc] = nah[r][kc];
+                al[r][kc] = nal[r][kc];
+            }
         lds_barrier();
     }
 
-    // ---- epilogue: L += acc / (scale of K * scale of X), in FP64
+    // ---- epilogue: L += acc / (scale of K * scale of X), in FP64 (or the
+    // part's partial sum, summed into L by a reduction, when split)
     const double inv = a.kinv / m16_xscale(*a.xmax);
+    double2* const dst = a.dsplit > 1 ? a.partial + (int64_t)part * a.size * tri(N) : a.dst;
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct) {
         uint32_t tx, ty, tz;
         const int t = target_of(ct * 32 + (lane & 31), tx, ty, tz);
         if (t < 0) continue;
-        double* out = (double*)(a.dst + (int64_t)t * tri(N));
+        double* out = (double*)(dst + (int64_t)t * tri(N));
 #pragma unroll
         for (int r = 0; r < RW; ++r) {
             const int rt = w + 4 * r;
@@ -218,7 +255,9 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
                 const int row = rt * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
                 if (row < NR) {
                     const int o = m16_coef_offset(row);
-                    out[o] += (double)acc[r][ct][v] * inv;
+                    const double val = (double)acc[r][ct][v] * inv;
+                    if (a.dsplit > 1) out[o] = val;
+                    else out[o] += val;
                 }
             }
         }
@@ -241,11 +280,12 @@ hipError_t do_m2l16_pack(const M2L16Args& a, hipStream_t st)
 template <int N>
 hipError_t do_m2l16(const M2L16Args& a, hipStream_t st)
 {
-    const int tc = 32 * m16_ct(N);
-    const int tiles = (a.num_parents + tc - 1) / tc;
+    static_assert(m16_targets(N) == 32 * m16_ct(N), "tile width");
+    const int tiles = (a.num_parents + m16_targets(N) - 1) / m16_targets(N);
+    if (tiles != a.tiles || a.dsplit < 1 || a.dsplit > kDirsPerClass) return hipErrorInvalidValue;
     if (tiles > 0)
-        hipLaunchKernelGGL((k_m2l16<N, m16_ct(N)>), dim3(tiles * 8), dim3(256), 0, st, a, a.kmat, (const uint4*)a.xp,
-                           a.class_dirs, a.dir_vec);
+        hipLaunchKernelGGL((k_m2l16<N, m16_ct(N)>), dim3(tiles * 8 * a.dsplit), dim3(256), 0, st, a, a.kmat,
+                           (const uint4*)a.xp, a.class_dirs, a.dir_vec);
     return hipGetLastError();
 }
 

@@ -1,0 +1,199 @@
+// ops2d.hip -- FP64 operator kernels of the 2D (complex-series) FMM, gfx950.
+//
+//   k_p2m2d    P2M, thread per leaf                    (reference Multipole, src/cpu/multipole.cpp:22-36)
+//   k_m2m2d    M2M, thread per (parent, coefficient)    (M2M :38-53, Upward :155-177)
+//   k_m2l2d    L2L + M2L, thread per (target, coefficient) (M2L :66-84, L2L :86-97, Downward :179-231)
+//   k_leaf2d   L2P + P2P over the 3x3 neighbourhood, thread per particle,
+//              results scattered to input order       (ComputePotentials / ComputeForces :233-449)
+//
+// Expansions are normalised per level (tables2d.cpp), so every operator is a
+// fixed complex matrix per integer box offset or child class.
+#include "device_common.h"
+#include "lfmm_internal.h"
+#include "ops2d.h"
+
+namespace lfmm {
+
+__device__ inline double2 cmadd(double2 acc, double2 a, double2 b)
+{
+    acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
+    return acc;
+}
+
+__global__ void __launch_bounds__(256) k_p2m2d(P2M2dArgs a)
+{
+    const int leaf = blockIdx.x * blockDim.x + threadIdx.x;
+    if (leaf >= a.nleaf) return;
+    const uint64_t key = a.leaf_key[leaf];
+    const double scale = a.geom[3];
+    const double cx = box_centre(compact2(key >> 1), a.level, scale, a.geom[0]);
+    const double cy = box_centre(compact2(key), a.level, scale, a.geom[1]);
+    const double ih = ldexp(1.0, a.level) / scale;
+    double2 acc[kMaxTerms2d];
+#pragma unroll
+    for (int k = 0; k < kMaxTerms2d; ++k) acc[k] = make_double2(0.0, 0.0);
+    const int b = a.leaf_begin[leaf], e = a.leaf_begin[leaf + 1];
+    for (int p = b; p < e; ++p) {
+        const double4 s = a.src[p];
+        const double ux = (s.x - cx) * ih, uy = (s.y - cy) * ih, q = s.w;
+        double wx = q * ux, wy = q * uy;  // q u^k
+        acc[0].x += q;
+#pragma unroll
+        for (int k = 1; k < kMaxTerms2d; ++k) {
+            if (k < a.N) {
+                acc[k].x += wx;
+                acc[k].y += wy;
+                const double t = wx * ux - wy * uy;
+                wy = wx * uy + wy * ux;
+                wx = t;
+            }
+        }
+    }
+    double2* out = a.outer + (int64_t)leaf * a.N;
+    out[0] = acc[0];
+#pragma unroll
+    for (int k = 1; k < kMaxTerms2d; ++k)
+        if (k < a.N) out[k] = make_double2(-acc[k].x / k, -acc[k].y / k);
+}
+
+__global__ void __launch_bounds__(256) k_m2m2d(M2M2dArgs a, const double2* __restrict__ mm)
+{
+    const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int N = a.N;
+    if (gid >= (int64_t)a.num_parents * N) return;
+    const int P = (int)(gid / N), j = (int)(gid % N);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int ch = a.child_begin[P]; ch < a.child_begin[P + 1]; ++ch) {
+        const int c = (int)(a.child_keys[ch] & 3u);
+        const double2* t = mm + ((int64_t)c * N + j) * N;
+        const double2* x = a.src + (int64_t)ch * N;
+        for (int s = 0; s <= j; ++s) acc = cmadd(acc, t[s], x[s]);
+    }
+    a.dst[(int64_t)P * N + j] = acc;
+}
+
+__global__ void __launch_bounds__(256) k_m2l2d(M2L2dArgs a, Tables2d T)
+{
+    const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int N = a.N;
+    if (gid >= (int64_t)a.num_parents * 4 * N) return;
+    const int l = (int)(gid % N);
+    const int64_t rest = gid / N;
+    const int c = (int)(rest & 3), P = (int)(rest >> 2);
+    int t = -1;
+    for (int i = a.child_begin[P]; i < a.child_begin[P + 1]; ++i)
+        if ((int)(a.child_keys[i] & 3u) == c) t = i;
+    if (t < 0) return;
+    const uint64_t key = a.child_keys[t];
+    const int tx = (int)compact2(key >> 1), ty = (int)compact2(key);
+    const int side = 1 << a.level;
+    double2 acc = make_double2(0.0, 0.0);
+    if (a.parent_local) {  // L2L from the parent
+        const double2* m = T.l2l + ((int64_t)c * N + l) * N;
+        const double2* x = a.parent_local + (int64_t)P * N;
+        for (int j = l; j < N; ++j) acc = cmadd(acc, m[j], x[j]);
+    }
+    for (int k = 0; k < kDirsPerClass2d; ++k) {
+        const int d = T.class_dirs[c * kDirsPerClass2d + k];
+        const int sx = tx + T.dir_vec[2 * d], sy = ty + T.dir_vec[2 * d + 1];
+        if (sx < 0 || sy < 0 || sx >= side || sy >= side) continue;
+        const int s = find_box(morton2(sx, sy), a.lut, a.keys, a.size);
+        if (s < 0) continue;
+        const double2* m = T.m2l + ((int64_t)d * N + l) * N;
+        const double2* x = a.src + (int64_t)s * N;
+        if (l == 0) {  // level term at_0 log h of the normalised M2L
+            acc.x = fma(x[0].x, a.log_h, acc.x);
+            acc.y = fma(x[0].y, a.log_h, acc.y);
+        }
+        for (int j = 0; j < N; ++j) acc = cmadd(acc, m[j], x[j]);
+    }
+    a.dst[(int64_t)t * N + l] = acc;
+}
+
+// Pair term of the reference 2D kernels (src/common/simple_math.h:119-128):
+// q log(max(|dz|, 1e-4)) and q conj(dz) / max(|dz|^2, 1e-8), self pair included.
+template <bool LOCAL>
+__global__ void __launch_bounds__(256) k_leaf2d(Leaf2dArgs a)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= a.n) return;
+    const int64_t o = a.perm[s];
+    if (o >= a.n_out) return;
+    const int leaf = a.leaf_of[s];
+    const uint64_t key = a.leaf_key[leaf];
+    const uint32_t ix = compact2(key >> 1), iy = compact2(key);
+    const double4 t = a.src[s];
+    double ph = 0.0, fx = 0.0, fy = 0.0;
+    if constexpr (LOCAL) {  // L2P by Horner: P(u) = sum bt_k u^k, P'(u)
+        const double scale = a.geom[3];
+        const double ih = ldexp(1.0, a.level) / scale;
+        const double ux = (t.x - box_centre(ix, a.level, scale, a.geom[0])) * ih;
+        const double uy = (t.y - box_centre(iy, a.level, scale, a.geom[1])) * ih;
+        const double2* b = a.local + (int64_t)leaf * a.N;
+        double2 p = b[a.N - 1], d = make_double2(0.0, 0.0);
+        for (int k = a.N - 2; k >= 0; --k) {
+            d = make_double2(d.x * ux - d.y * uy + p.x, d.x * uy + d.y * ux + p.y);
+            const double2 bk = b[k];
+            p = make_double2(p.x * ux - p.y * uy + bk.x, p.x * uy + p.y * ux + bk.y);
+        }
+        ph = p.x;
+        fx = d.x * ih;
+        fy = d.y * ih;
+    }
+    const int side = 1 << a.level;
+    for (int nb9 = 0; nb9 < 9; ++nb9) {
+        const int nx = (int)ix + nb9 / 3 - 1, ny = (int)iy + nb9 % 3 - 1;
+        if (nx < 0 || ny < 0 || nx >= side || ny >= side) continue;
+        const int nb = (nb9 == 4) ? leaf : find_box(morton2(nx, ny), a.lut, a.leaf_key, a.nleaf);
+        if (nb < 0) continue;
+        for (int j = a.leaf_begin[nb]; j < a.leaf_begin[nb + 1]; ++j) {
+            if (a.exclude_self && j == (int)s) continue;
+            const double4 p = a.src[j];
+            const double dx = t.x - p.x, dy = t.y - p.y;
+            const double nz = fmax(dx * dx + dy * dy, 1.e-8);
+            const double qi = p.w / nz;
+            ph += p.w * 0.5 * log(nz);
+            fx = fma(qi, dx, fx);
+            fy = fma(-qi, dy, fy);
+        }
+    }
+    if (a.pot) a.pot[o] = ph;
+    if (a.field) {
+        a.field[2 * o] = fx;
+        a.field[2 * o + 1] = fy;
+    }
+}
+
+static inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+hipError_t launch_p2m2d(const P2M2dArgs& a, hipStream_t st)
+{
+    if (a.N < 1 || a.N > kMaxTerms2d) return hipErrorInvalidValue;
+    if (a.nleaf > 0) hipLaunchKernelGGL(k_p2m2d, dim3(nblk(a.nleaf, 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_m2m2d(const M2M2dArgs& a, const Tables2d& T, hipStream_t st)
+{
+    const int64_t n = (int64_t)a.num_parents * a.N;
+    if (n > 0) hipLaunchKernelGGL(k_m2m2d, dim3(nblk(n, 256)), dim3(256), 0, st, a, T.m2m);
+    return hipGetLastError();
+}
+
+hipError_t launch_m2l2d(const M2L2dArgs& a, const Tables2d& T, hipStream_t st)
+{
+    const int64_t n = (int64_t)a.num_parents * 4 * a.N;
+    if (n > 0) hipLaunchKernelGGL(k_m2l2d, dim3(nblk(n, 256)), dim3(256), 0, st, a, T);
+    return hipGetLastError();
+}
+
+hipError_t launch_leaf2d(const Leaf2dArgs& a, hipStream_t st)
+{
+    if (a.n <= 0) return hipSuccess;
+    if (a.local) hipLaunchKernelGGL(k_leaf2d<true>, dim3(nblk(a.n, 256)), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_leaf2d<false>, dim3(nblk(a.n, 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace lfmm

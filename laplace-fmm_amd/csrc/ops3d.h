@@ -26,6 +26,14 @@ struct RtrTables {
     const int* class_dirs;     // [8][189]
 };
 
+// Targets per workgroup of the rotate-translate-rotate kernel: 64 (one per
+// lane) while the FP64 op state of 64 targets fits the 160 KiB LDS, fewer
+// lanes above that (N = 18).
+constexpr int rtr_tile(int N)
+{
+    return (N * (N + 1) / 2) * 64 * 16 <= 160 * 1024 ? 64 : (160 * 1024) / ((N * (N + 1) / 2) * 16) / 8 * 8;
+}
+
 struct RtrLevel {
     int level;                 // M2L: target level; M2M: parent level
     int num_parents;           // M2L: boxes at level-1; M2M: boxes at level
@@ -37,6 +45,12 @@ struct RtrLevel {
     const double2* src;        // M2L: outer[level]; M2M: outer[level+1]
     const double2* parent_local;  // M2L: inner[level-1] (null: no L2L)
     double2* dst;              // M2L: inner[level]; M2M: outer[level]
+    // M2L / L2L: target tiles (ceil(num_parents / rtr_tile(N))) and the number
+    // of direction parts; with dsplit > 1 part p writes partial + p*size*NX
+    // and a reduction forms dst (small levels: more workgroups than tiles).
+    int tiles = 0;
+    int dsplit = 1;
+    double2* partial = nullptr;
 };
 
 struct LeafArgs {
@@ -75,13 +89,22 @@ struct M2L16Args {
     int size;                  // boxes at `level`
     double kinv;               // 1 / scale of the packed operators
     double2* dst;              // locals of the level (+= M2L)
+    int tiles = 0;             // target tiles of 32 * m16 column tiles
+    int dsplit = 1;            // direction parts (partial + p*size*tri(N) when > 1)
+    double2* partial = nullptr;
 };
+
+// Targets per workgroup of the matrix-core M2L kernel (two column tiles of
+// 32 while the double-buffered source stage fits the LDS, one above N = 16).
+constexpr int m16_targets(int N) { return N <= 16 ? 64 : 32; }
 
 hipError_t launch_p2m(int N, const P2MArgs& a, hipStream_t st);
 hipError_t launch_rtr(int N, int kind, const RtrTables& T, const RtrLevel& L, hipStream_t st);
 hipError_t launch_leaf(int N, bool local, const LeafArgs& a, hipStream_t st);
 hipError_t launch_m2l16_pack(int N, const M2L16Args& a, hipStream_t st);
 hipError_t launch_m2l16(int N, const M2L16Args& a, hipStream_t st);
+// dst[i] = (add ? dst[i] : 0) + sum_p partial[p * count + i]   (m2l16.hip)
+hipError_t launch_sum_parts(const double2* partial, int parts, int64_t count, double2* dst, bool add, hipStream_t st);
 // *out = max(*out, largest |v[i]|) as float bits (m2l16.hip)
 hipError_t launch_absmax(const double* v, int64_t count, uint32_t* out, hipStream_t st);
 

@@ -145,9 +145,7 @@ __global__ void __launch_bounds__(256) k_p2m(P2MArgs a, const double* __restrict
 // ===================================================================== rotate-translate-rotate
 enum { KIND_M2M = 0, KIND_M2L = 1, KIND_L2L = 2 };  // KIND_L2L: the L2L step of KIND_M2L alone
 
-// Targets per workgroup: 64 (one per lane) while the FP64 op state of 64
-// targets fits the 160 KiB LDS, fewer lanes above that (N = 18).
-constexpr int rtr_tile(int N) { return tri(N) * 64 * 16 <= 160 * 1024 ? 64 : (160 * 1024) / (tri(N) * 16) / 8 * 8; }
+// rtr_tile(N) (targets per workgroup) is declared in ops3d.h.
 enum { TR_M2L = 0, TR_M2M = 1, TR_L2L = 2 };
 
 template <int N, int W, int TK>
@@ -240,10 +238,15 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
     constexpr int TW = rtr_tile(N);
     const bool lok = lane < TW;
     const int blk = xcd_local_block(blockIdx.x, gridDim.x);
-    int tile, c;
+    // class-major block order: the workgroups resident on one XCD share a child
+    // class (and direction part), so they walk the same directions together
+    int tile, c, part = 0;
     if constexpr (KIND != KIND_M2M) {
-        tile = blk >> 3;
-        c = blk & 7;
+        const int per_class = L.tiles * L.dsplit;
+        c = blk / per_class;
+        const int r = blk - c * per_class;
+        part = r / L.tiles;
+        tile = r - part * L.tiles;
     } else {
         tile = blk;
         c = 0;
@@ -276,6 +279,8 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
 
     const int has_l2l = (KIND == KIND_L2L || (KIND == KIND_M2L && L.parent_local)) ? 1 : 0;
     const int ndir = (KIND == KIND_M2M) ? 8 : (KIND == KIND_L2L ? 1 : kDirsPerClass + has_l2l);
+    // directions [d_beg, d_end) of this part (L.dsplit parts; the L2L step is in part 0)
+    const int d_beg = (int)((int64_t)part * ndir / L.dsplit), d_end = (int)((int64_t)(part + 1) * ndir / L.dsplit);
 
     auto step_of = [&](int d) {
         RtrStep s;
@@ -321,9 +326,9 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
         });
     };
 
-    RtrStep nxt = step_of(0);
+    RtrStep nxt = step_of(d_beg);
     prefetch(nxt);
-    for (int d = 0; d < ndir; ++d) {
+    for (int d = d_beg; d < d_end; ++d) {
         const RtrStep cur = nxt;
         const bool any = __ballot(cur.sidx >= 0) != 0ull;  // same lanes in every wave: uniform
         const double2* __restrict__ rot = Trot + (int64_t)Tdir_theta[cur.dir] * RS;
@@ -369,7 +374,7 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
             lds_barrier();
         }
         // issue the next step's source loads; they complete under phase C
-        if (d + 1 < ndir) {
+        if (d + 1 < d_end) {
             nxt = step_of(d + 1);
             prefetch(nxt);
         }
@@ -409,7 +414,7 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
         }
     }
     if (tgt >= 0) {
-        double2* out = L.dst + (int64_t)tgt * NX;
+        double2* out = (L.dsplit > 1 ? L.partial + (int64_t)part * L.size * NX : L.dst) + (int64_t)tgt * NX;
         sfor<0, N>([&](auto JJ) {
             constexpr int j = decltype(JJ)::value;
             if constexpr (PT.deg[j] == W) {
@@ -603,15 +608,16 @@ hipError_t do_rtr(int kind, const RtrTables& T, const RtrLevel& L, hipStream_t s
 {
     const int tiles = (L.num_parents + rtr_tile(N) - 1) / rtr_tile(N);
     if (tiles == 0) return hipSuccess;
+    if (L.tiles != tiles && kind != KIND_M2M) return hipErrorInvalidValue;
     if (kind == KIND_M2M)
         hipLaunchKernelGGL((k_rtr<N, KIND_M2M>), dim3(tiles), dim3(256), 0, st, T.rot, T.ephi, T.trans, T.dir_theta,
                            T.dir_trans, T.dir_vec, T.class_dirs, L);
     else if (kind == KIND_L2L)
-        hipLaunchKernelGGL((k_rtr<N, KIND_L2L>), dim3(tiles * 8), dim3(256), 0, st, T.rot, T.ephi, T.trans, T.dir_theta,
-                           T.dir_trans, T.dir_vec, T.class_dirs, L);
+        hipLaunchKernelGGL((k_rtr<N, KIND_L2L>), dim3(tiles * 8 * L.dsplit), dim3(256), 0, st, T.rot, T.ephi, T.trans,
+                           T.dir_theta, T.dir_trans, T.dir_vec, T.class_dirs, L);
     else
-        hipLaunchKernelGGL((k_rtr<N, KIND_M2L>), dim3(tiles * 8), dim3(256), 0, st, T.rot, T.ephi, T.trans, T.dir_theta,
-                           T.dir_trans, T.dir_vec, T.class_dirs, L);
+        hipLaunchKernelGGL((k_rtr<N, KIND_M2L>), dim3(tiles * 8 * L.dsplit), dim3(256), 0, st, T.rot, T.ephi, T.trans,
+                           T.dir_theta, T.dir_trans, T.dir_vec, T.class_dirs, L);
     return hipGetLastError();
 }
 

@@ -18,13 +18,15 @@
 namespace lfmm {
 
 // ------------------------------------------------------------------ extent
+// D = 2 or 3 coordinates per point; unused axes (z in 2D) report 0.
+template <int D>
 __global__ void __launch_bounds__(256) k_extent_partial(const double* __restrict__ pos, int64_t stride, int64_t n,
                                                         double* __restrict__ part, int* __restrict__ nonfinite)
 {
-    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double mn[3] = {INFINITY, INFINITY, D == 3 ? INFINITY : 0.0}, mx[3] = {-INFINITY, -INFINITY, D == 3 ? -INFINITY : 0.0};
     bool bad = false;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        for (int a = 0; a < 3; ++a) {
+        for (int a = 0; a < D; ++a) {
             const double v = pos[i * stride + a];
             bad |= !isfinite(v);
             mn[a] = fmin(mn[a], v);
@@ -87,34 +89,39 @@ __device__ inline uint32_t grid_coord(double p, double shift, double scale, int 
     return (uint32_t)(uint64_t)((double)(1ull << bits) * x);
 }
 
+template <int D>
+__device__ inline uint64_t point_key(const double* p, const double* geom, int bits)
+{
+    const uint32_t ix = grid_coord(p[0], geom[0], geom[3], bits);
+    const uint32_t iy = grid_coord(p[1], geom[1], geom[3], bits);
+    if constexpr (D == 2) return morton2(ix, iy);
+    else return morton3(ix, iy, grid_coord(p[2], geom[2], geom[3], bits));
+}
+
+template <int D>
 __global__ void __launch_bounds__(256) k_keys(const double* __restrict__ pos, int64_t stride, int64_t n, int bits,
                                               const double* __restrict__ geom, uint64_t* __restrict__ keys,
                                               uint32_t* __restrict__ vals)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double sx = geom[0], sy = geom[1], sz = geom[2], sc = geom[3];
-    const uint32_t ix = grid_coord(pos[i * stride], sx, sc, bits);
-    const uint32_t iy = grid_coord(pos[i * stride + 1], sy, sc, bits);
-    const uint32_t iz = grid_coord(pos[i * stride + 2], sz, sc, bits);
-    keys[i] = morton3(ix, iy, iz);
+    keys[i] = point_key<D>(pos + i * stride, geom, bits);
     vals[i] = (uint32_t)i;
 }
 
 // Occupancy histogram at a fixed fine level (used to choose the depth from a
 // leaf-size bound): counts[key] += 1.
+template <int D>
 __global__ void __launch_bounds__(256) k_occupancy(const double* __restrict__ pos, int64_t stride, int64_t n, int bits,
                                                    const double* __restrict__ geom, uint32_t* __restrict__ counts)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double sx = geom[0], sy = geom[1], sz = geom[2], sc = geom[3];
-    const uint64_t k = morton3(grid_coord(pos[i * stride], sx, sc, bits), grid_coord(pos[i * stride + 1], sy, sc, bits),
-                               grid_coord(pos[i * stride + 2], sz, sc, bits));
-    atomicAdd(&counts[k], 1u);
+    atomicAdd(&counts[point_key<D>(pos + i * stride, geom, bits)], 1u);
 }
 
-// counts at level l-1 from level l (8 children each) and the level maximum.
+// counts at level l-1 from level l (2^D children each) and the level maximum.
+template <int D>
 __global__ void __launch_bounds__(256) k_occupancy_up(const uint32_t* __restrict__ fine, int64_t ncoarse,
                                                       uint32_t* __restrict__ coarse, uint32_t* __restrict__ maxv)
 {
@@ -122,7 +129,7 @@ __global__ void __launch_bounds__(256) k_occupancy_up(const uint32_t* __restrict
     uint32_t m = 0;
     if (i < ncoarse) {
         uint32_t s = 0;
-        for (int c = 0; c < 8; ++c) s += fine[i * 8 + c];
+        for (int c = 0; c < (1 << D); ++c) s += fine[i * (1 << D) + c];
         coarse[i] = s;
         m = s;
     }
@@ -336,13 +343,15 @@ __global__ void __launch_bounds__(kRsThreads) k_rs_scatter(const uint64_t* __res
 // Packs a strided point view into (x, y, z, q) records at out[offset + i]
 // (q = 0 when no charges are given: the reference marks targets by q = 0,
 // src/examples/main.cpp:125-126).
+// (x, y, z, q) records; z = 0 in 2D.
+template <int D>
 __global__ void __launch_bounds__(256) k_pack_points(const double* __restrict__ pos, int64_t ps,
                                                      const double* __restrict__ q, int64_t qs, int64_t n,
                                                      double4* __restrict__ out)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    out[i] = make_double4(pos[i * ps], pos[i * ps + 1], pos[i * ps + 2], q ? q[i * qs] : 0.0);
+    out[i] = make_double4(pos[i * ps], pos[i * ps + 1], D == 3 ? pos[i * ps + 2] : 0.0, q ? q[i * qs] : 0.0);
 }
 
 // Source/target partition inside each leaf (src/cpu/morton_tree.cpp:437-452):
@@ -368,6 +377,7 @@ __global__ void __launch_bounds__(256) k_partition_leaves(const int32_t* __restr
 
 // ------------------------------------------------------------------ gather
 // Sorted sources as (x, y, z, q) double4 plus the sorted->original map.
+template <int D>
 __global__ void __launch_bounds__(256) k_gather(const double* __restrict__ pos, int64_t pstride,
                                                 const double* __restrict__ q, int64_t qstride,
                                                 const uint32_t* __restrict__ order, int64_t n,
@@ -376,7 +386,7 @@ __global__ void __launch_bounds__(256) k_gather(const double* __restrict__ pos, 
     const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int64_t i = order[s];
-    src[s] = make_double4(pos[i * pstride], pos[i * pstride + 1], pos[i * pstride + 2], q[i * qstride]);
+    src[s] = make_double4(pos[i * pstride], pos[i * pstride + 1], D == 3 ? pos[i * pstride + 2] : 0.0, q[i * qstride]);
 }
 
 // ------------------------------------------------------------------ levels
@@ -444,6 +454,7 @@ __device__ inline int lookup(uint64_t key, const int32_t* lut, const uint64_t* k
     return (lo < size && keys[lo] == key) ? lo : -1;
 }
 
+template <int D>
 __global__ void __launch_bounds__(256) k_count_m2l(const uint64_t* __restrict__ keys, int size, int level,
                                                    const int32_t* __restrict__ lut, unsigned long long* __restrict__ out)
 {
@@ -451,22 +462,25 @@ __global__ void __launch_bounds__(256) k_count_m2l(const uint64_t* __restrict__ 
     unsigned long long c = 0;
     if (i < size) {
         const uint64_t k = keys[i];
-        const int x = compact3(k >> 2), y = compact3(k >> 1), z = compact3(k);
+        const int x = D == 3 ? compact3(k >> 2) : compact2(k >> 1), y = D == 3 ? compact3(k >> 1) : compact2(k),
+                  z = D == 3 ? compact3(k) : 0;
         const int side = 1 << level;
-        const int bx = (x & ~1) - 2, by = (y & ~1) - 2, bz = (z & ~1) - 2;
+        const int bx = (x & ~1) - 2, by = (y & ~1) - 2, bz = D == 3 ? (z & ~1) - 2 : 0;
         for (int a = 0; a < 6; ++a)
             for (int b = 0; b < 6; ++b)
-                for (int d = 0; d < 6; ++d) {
+                for (int d = 0; d < (D == 3 ? 6 : 1); ++d) {
                     const int sx = bx + a, sy = by + b, sz = bz + d;
                     if (sx < 0 || sy < 0 || sz < 0 || sx >= side || sy >= side || sz >= side) continue;
                     if (abs(sx - x) <= 1 && abs(sy - y) <= 1 && abs(sz - z) <= 1) continue;
-                    if (lookup(morton3(sx, sy, sz), lut, keys, size) >= 0) ++c;
+                    const uint64_t sk = D == 3 ? morton3(sx, sy, sz) : morton2(sx, sy);
+                    if (lookup(sk, lut, keys, size) >= 0) ++c;
                 }
     }
     for (int off = 32; off > 0; off >>= 1) c += (unsigned long long)__shfl_xor((long long)c, off);
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
 }
 
+template <int D>
 __global__ void __launch_bounds__(256) k_count_p2p(const uint64_t* __restrict__ keys, const int32_t* __restrict__ begin,
                                                    int size, int level, const int32_t* __restrict__ lut,
                                                    unsigned long long* __restrict__ out)
@@ -475,13 +489,15 @@ __global__ void __launch_bounds__(256) k_count_p2p(const uint64_t* __restrict__ 
     unsigned long long c = 0;
     if (i < size) {
         const uint64_t k = keys[i];
-        const int x = compact3(k >> 2), y = compact3(k >> 1), z = compact3(k);
+        const int x = D == 3 ? compact3(k >> 2) : compact2(k >> 1), y = D == 3 ? compact3(k >> 1) : compact2(k),
+                  z = D == 3 ? compact3(k) : 0;
         const int side = 1 << level;
         unsigned long long s = 0;
-        for (int o = 0; o < 27; ++o) {
-            const int sx = x + o / 9 - 1, sy = y + (o / 3) % 3 - 1, sz = z + o % 3 - 1;
+        for (int o = 0; o < (D == 3 ? 27 : 9); ++o) {
+            const int sx = x + (D == 3 ? o / 9 : o / 3) - 1, sy = y + (D == 3 ? (o / 3) % 3 : o % 3) - 1,
+                      sz = D == 3 ? z + o % 3 - 1 : 0;
             if (sx < 0 || sy < 0 || sz < 0 || sx >= side || sy >= side || sz >= side) continue;
-            const int j = lookup(morton3(sx, sy, sz), lut, keys, size);
+            const int j = lookup(D == 3 ? morton3(sx, sy, sz) : morton2(sx, sy), lut, keys, size);
             if (j >= 0) s += (unsigned long long)(begin[j + 1] - begin[j]);
         }
         c = s * (unsigned long long)(begin[i + 1] - begin[i]);
@@ -493,40 +509,52 @@ __global__ void __launch_bounds__(256) k_count_p2p(const uint64_t* __restrict__ 
 // ================================================================== host side
 static inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
-hipError_t extent(const double* pos, int64_t stride, int64_t n, double* part, int* nonfinite, double* geom, hipStream_t st)
+hipError_t extent(int dim, const double* pos, int64_t stride, int64_t n, double* part, int* nonfinite, double* geom,
+                  hipStream_t st)
 {
     const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, nblk(n, 256)));
-    hipLaunchKernelGGL(k_extent_partial, dim3(nb), dim3(256), 0, st, pos, stride, n, part, nonfinite);
+    if (dim == 2) hipLaunchKernelGGL(k_extent_partial<2>, dim3(nb), dim3(256), 0, st, pos, stride, n, part, nonfinite);
+    else hipLaunchKernelGGL(k_extent_partial<3>, dim3(nb), dim3(256), 0, st, pos, stride, n, part, nonfinite);
     hipLaunchKernelGGL(k_extent_final, dim3(1), dim3(256), 0, st, part, nb, geom);
     return hipGetLastError();
 }
 
-hipError_t keys(const double* pos, int64_t stride, int64_t n, int bits, const double* geom, uint64_t* k, uint32_t* v,
-                hipStream_t st)
+hipError_t keys(int dim, const double* pos, int64_t stride, int64_t n, int bits, const double* geom, uint64_t* k,
+                uint32_t* v, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_keys, dim3(nblk(n, 256)), dim3(256), 0, st, pos, stride, n, bits, geom, k, v);
+    if (dim == 2) hipLaunchKernelGGL(k_keys<2>, dim3(nblk(n, 256)), dim3(256), 0, st, pos, stride, n, bits, geom, k, v);
+    else hipLaunchKernelGGL(k_keys<3>, dim3(nblk(n, 256)), dim3(256), 0, st, pos, stride, n, bits, geom, k, v);
     return hipGetLastError();
 }
 
-hipError_t occupancy_depth(const double* pos, int64_t stride, int64_t n, const double* geom, int leaf_max, int max_level,
-                           uint32_t* counts_ws, uint32_t* maxv_ws, int* level_out, hipStream_t st)
+hipError_t occupancy_depth(int dim, const double* pos, int64_t stride, int64_t n, const double* geom, int leaf_max,
+                           int max_level, uint32_t* counts_ws, uint32_t* maxv_ws, int* level_out, hipStream_t st)
 {
     // histogram at max_level, then coarsen level by level; pick the smallest
     // level whose fullest box holds <= leaf_max particles.
-    const int64_t nb = (int64_t)1 << (3 * max_level);
+    const int64_t nb = (int64_t)1 << (dim * max_level);
     int64_t off = 0;
     std::vector<int64_t> offs(max_level + 1);
-    for (int l = max_level; l >= 0; --l) { offs[l] = off; off += (int64_t)1 << (3 * l); }
+    for (int l = max_level; l >= 0; --l) { offs[l] = off; off += (int64_t)1 << (dim * l); }
     hipError_t e = hipMemsetAsync(counts_ws, 0, sizeof(uint32_t) * nb, st);
     if (e == hipSuccess) e = hipMemsetAsync(maxv_ws, 0, sizeof(uint32_t) * (max_level + 1), st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_occupancy, dim3(nblk(n, 256)), dim3(256), 0, st, pos, stride, n, max_level, geom, counts_ws + offs[max_level]);
+    if (dim == 2)
+        hipLaunchKernelGGL(k_occupancy<2>, dim3(nblk(n, 256)), dim3(256), 0, st, pos, stride, n, max_level, geom,
+                           counts_ws + offs[max_level]);
+    else
+        hipLaunchKernelGGL(k_occupancy<3>, dim3(nblk(n, 256)), dim3(256), 0, st, pos, stride, n, max_level, geom,
+                           counts_ws + offs[max_level]);
     hipLaunchKernelGGL(k_max_u32, dim3((int)std::min<int64_t>(1024, nblk(nb, 256))), dim3(256), 0, st,
                        counts_ws + offs[max_level], nb, maxv_ws + max_level);
     for (int l = max_level - 1; l >= 0; --l) {
-        const int64_t nc = (int64_t)1 << (3 * l);
-        hipLaunchKernelGGL(k_occupancy_up, dim3(nblk(nc, 256)), dim3(256), 0, st, counts_ws + offs[l + 1], nc,
-                           counts_ws + offs[l], maxv_ws + l);
+        const int64_t nc = (int64_t)1 << (dim * l);
+        if (dim == 2)
+            hipLaunchKernelGGL(k_occupancy_up<2>, dim3(nblk(nc, 256)), dim3(256), 0, st, counts_ws + offs[l + 1], nc,
+                               counts_ws + offs[l], maxv_ws + l);
+        else
+            hipLaunchKernelGGL(k_occupancy_up<3>, dim3(nblk(nc, 256)), dim3(256), 0, st, counts_ws + offs[l + 1], nc,
+                               counts_ws + offs[l], maxv_ws + l);
     }
     std::vector<uint32_t> mx(max_level + 1);
     e = hipMemcpyAsync(mx.data(), maxv_ws, sizeof(uint32_t) * (max_level + 1), hipMemcpyDeviceToHost, st);
@@ -584,9 +612,11 @@ hipError_t radix_sort(uint64_t* keys, uint32_t* vals, int64_t n, int bits, void*
     return hipGetLastError();
 }
 
-hipError_t pack_points(const double* pos, int64_t ps, const double* q, int64_t qs, int64_t n, double4* out, hipStream_t st)
+hipError_t pack_points(int dim, const double* pos, int64_t ps, const double* q, int64_t qs, int64_t n, double4* out,
+                       hipStream_t st)
 {
-    if (n > 0) hipLaunchKernelGGL(k_pack_points, dim3(nblk(n, 256)), dim3(256), 0, st, pos, ps, q, qs, n, out);
+    if (n > 0 && dim == 2) hipLaunchKernelGGL(k_pack_points<2>, dim3(nblk(n, 256)), dim3(256), 0, st, pos, ps, q, qs, n, out);
+    else if (n > 0) hipLaunchKernelGGL(k_pack_points<3>, dim3(nblk(n, 256)), dim3(256), 0, st, pos, ps, q, qs, n, out);
     return hipGetLastError();
 }
 
@@ -597,10 +627,11 @@ hipError_t partition_leaves(const int32_t* begin, int64_t nleaf, uint32_t* order
     return hipGetLastError();
 }
 
-hipError_t gather_sources(const double* pos, int64_t pstride, const double* q, int64_t qstride, const uint32_t* order,
-                          int64_t n, double4* src, hipStream_t st)
+hipError_t gather_sources(int dim, const double* pos, int64_t pstride, const double* q, int64_t qstride,
+                          const uint32_t* order, int64_t n, double4* src, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_gather, dim3(nblk(n, 256)), dim3(256), 0, st, pos, pstride, q, qstride, order, n, src);
+    if (dim == 2) hipLaunchKernelGGL(k_gather<2>, dim3(nblk(n, 256)), dim3(256), 0, st, pos, pstride, q, qstride, order, n, src);
+    else hipLaunchKernelGGL(k_gather<3>, dim3(nblk(n, 256)), dim3(256), 0, st, pos, pstride, q, qstride, order, n, src);
     return hipGetLastError();
 }
 
@@ -621,27 +652,36 @@ hipError_t run_compact(const uint64_t* keys, int64_t m, int sh, uint32_t* flags,
     return hipGetLastError();
 }
 
-hipError_t build_lut(const uint64_t* ukeys, int64_t count, int level, int32_t* lut, hipStream_t st)
+hipError_t build_lut(int dim, const uint64_t* ukeys, int64_t count, int level, int32_t* lut, hipStream_t st)
 {
-    const int64_t span = (int64_t)1 << (3 * level);
+    const int64_t span = (int64_t)1 << (dim * level);
     hipLaunchKernelGGL(k_fill_i32, dim3((int)std::min<int64_t>(4096, nblk(span, 256))), dim3(256), 0, st, lut, span, -1);
     hipLaunchKernelGGL(k_lut_fill, dim3(nblk(count, 256)), dim3(256), 0, st, ukeys, count, lut);
     return hipGetLastError();
 }
 
-hipError_t count_ops(const std::vector<uint64_t*>& keys, const std::vector<int32_t*>& begin,
+hipError_t count_ops(int dim, const std::vector<uint64_t*>& keys, const std::vector<int32_t*>& begin,
                      const std::vector<int32_t*>& lut, const std::vector<int64_t>& cnt, unsigned long long* ws,
                      int64_t* m2l_ops, int64_t* p2p_pairs, int64_t* m2l_per_level, hipStream_t st)
 {
     const int depth = (int)cnt.size();
     hipError_t e = hipMemsetAsync(ws, 0, (2 + depth) * sizeof(unsigned long long), st);
     if (e != hipSuccess) return e;
-    for (int l = 2; l < depth; ++l)
-        hipLaunchKernelGGL(k_count_m2l, dim3(nblk(cnt[l], 256)), dim3(256), 0, st, keys[l], (int)cnt[l], l, lut[l],
-                           ws + 2 + l);
+    for (int l = 2; l < depth; ++l) {
+        if (dim == 2)
+            hipLaunchKernelGGL(k_count_m2l<2>, dim3(nblk(cnt[l], 256)), dim3(256), 0, st, keys[l], (int)cnt[l], l, lut[l],
+                               ws + 2 + l);
+        else
+            hipLaunchKernelGGL(k_count_m2l<3>, dim3(nblk(cnt[l], 256)), dim3(256), 0, st, keys[l], (int)cnt[l], l, lut[l],
+                               ws + 2 + l);
+    }
     const int L = depth - 1;
-    hipLaunchKernelGGL(k_count_p2p, dim3(nblk(cnt[L], 256)), dim3(256), 0, st, keys[L], begin[L], (int)cnt[L], L,
-                       L >= 2 ? lut[L] : nullptr, ws + 1);
+    if (dim == 2)
+        hipLaunchKernelGGL(k_count_p2p<2>, dim3(nblk(cnt[L], 256)), dim3(256), 0, st, keys[L], begin[L], (int)cnt[L], L,
+                           L >= 2 ? lut[L] : nullptr, ws + 1);
+    else
+        hipLaunchKernelGGL(k_count_p2p<3>, dim3(nblk(cnt[L], 256)), dim3(256), 0, st, keys[L], begin[L], (int)cnt[L], L,
+                           L >= 2 ? lut[L] : nullptr, ws + 1);
     std::vector<unsigned long long> h(2 + depth);
     e = hipMemcpyAsync(h.data(), ws, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);

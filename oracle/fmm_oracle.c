@@ -794,6 +794,13 @@ orc_tree3d* orc_tree3d_build(int64_t n, const double* xyz, const double* q, int 
 	return tree_build(3, n, xyz, q, depth);
 }
 
+/* MortonTree<2> (src/cpu/morton_tree.cpp with dim = 2); read back through the
+ * orc_tree3d_* accessors, which follow the tree's dimension. */
+orc_tree3d* orc_tree2d_build(int64_t n, const double* xy, const double* q, int depth)
+{
+	return tree_build(2, n, xy, q, depth);
+}
+
 void orc_tree3d_free(orc_tree3d* t)
 {
 	if (!t) return;

@@ -61,6 +61,7 @@ void orc_l2p3d(int N, const double* L, const double* centre, int64_t n, const do
 /* ---- tree (morton_tree.cpp:48-493) ---- */
 typedef struct orc_tree3d orc_tree3d;
 orc_tree3d* orc_tree3d_build(int64_t n, const double* xyz, const double* q, int depth);
+orc_tree3d* orc_tree2d_build(int64_t n, const double* xy, const double* q, int depth);
 void orc_tree3d_free(orc_tree3d* t);
 int  orc_tree3d_depth(const orc_tree3d* t);
 void orc_tree3d_info(const orc_tree3d* t, int64_t* level_sizes, double* shift_scale, int64_t* targets_num);

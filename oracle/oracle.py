@@ -37,6 +37,8 @@ def lib():
         L.orc_m2lcoef.argtypes = [ctypes.c_int] * 3
         L.orc_tree3d_build.restype = ctypes.c_void_p
         L.orc_tree3d_build.argtypes = [ctypes.c_int64, _d, _d, ctypes.c_int]
+        L.orc_tree2d_build.restype = ctypes.c_void_p
+        L.orc_tree2d_build.argtypes = [ctypes.c_int64, _d, _d, ctypes.c_int]
         L.orc_tree3d_free.argtypes = [ctypes.c_void_p]
         L.orc_tree3d_depth.argtypes = [ctypes.c_void_p]
         L.orc_tree3d_info.argtypes = [ctypes.c_void_p, _i64, _d, _i64]
@@ -166,12 +168,15 @@ def l2p3d(N, L, centre, xyz):
 # ---- tree ---------------------------------------------------------------
 class Tree3d:
     """The reference MortonTree<3> (stable order), as numpy arrays."""
+    dim = 3
 
     def __init__(self, xyz, q, depth):
-        xyz = _f64(xyz).reshape(-1, 3)
+        D = self.dim
+        xyz = _f64(xyz).reshape(-1, D)
         q = _f64(q)
         self.n = len(q)
-        h = lib().orc_tree3d_build(self.n, _p(xyz), _p(q), depth)
+        build = lib().orc_tree3d_build if D == 3 else lib().orc_tree2d_build
+        h = build(self.n, _p(xyz), _p(q), depth)
         if not h:
             raise ValueError("oracle tree build failed")
         try:
@@ -181,12 +186,12 @@ class Tree3d:
             tn = np.zeros(1, dtype=np.int64)
             lib().orc_tree3d_info(h, _p(sizes, ctypes.c_int64), _p(ss), _p(tn, ctypes.c_int64))
             self.level_sizes = sizes
-            self.shift = ss[:3].copy()
-            self.scale = float(ss[3])
+            self.shift = ss[:D].copy()
+            self.scale = float(ss[D])
             self.targets_num = int(tn[0])
             self.keys = np.zeros(self.n, dtype=np.uint64)
             self.pos_map = np.zeros(self.n, dtype=np.int64)
-            self.xyzq = np.zeros((self.n, 4))
+            self.xyzq = np.zeros((self.n, D + 1))
             lib().orc_tree3d_particles(h, _p(self.keys, ctypes.c_uint64), _p(self.pos_map, ctypes.c_int64),
                                        _p(self.xyzq))
             self.levels = []
@@ -194,7 +199,7 @@ class Tree3d:
                 s = int(sizes[l])
                 lv = dict(keys=np.zeros(s, dtype=np.uint64), src=np.zeros((s, 2), dtype=np.int64),
                           tgt=np.zeros((s, 2), dtype=np.int64), parent=np.zeros(s, dtype=np.int64),
-                          centre=np.zeros((s, 3)))
+                          centre=np.zeros((s, D)))
                 lib().orc_tree3d_level(h, l, _p(lv["keys"], ctypes.c_uint64), _p(lv["src"], ctypes.c_int64),
                                        _p(lv["tgt"], ctypes.c_int64), _p(lv["parent"], ctypes.c_int64),
                                        _p(lv["centre"]))
@@ -225,6 +230,11 @@ def fmm3d(src_xyz, src_q, N, depth, tgt_xyz=None, eps=1e-8, timings=False):
     if rc != 0:
         raise RuntimeError("oracle fmm3d failed: %d" % rc)
     return (pot, field, tm) if timings else (pot, field)
+
+
+class Tree2d(Tree3d):
+    """The reference MortonTree<2> (stable order), as numpy arrays."""
+    dim = 2
 
 
 def fmm2d(src_xy, src_q, N, depth, tgt_xy=None, eps=1e-8, timings=False):

@@ -51,15 +51,22 @@ def test_cpp_dropin_header_compiles_and_links(lfmm, tmp_path):
     src = tmp_path / "use.cpp"
     src.write_text(r'''
 #include "fmm/multipole3d.h"
+#include "fmm/multipole.h"
 #include <cstdio>
 int main() {
     std::vector<fmm::particle3d> p(3);
     for (int i = 0; i < 3; ++i) { p[i].center = fmm::Vector3d(0.1 * i, 0.2, 0.3); p[i].q = 1.0; }
+    std::vector<fmm::particle2d> p2(3);
+    for (int i = 0; i < 3; ++i) { p2[i].center = fmm::point2d(0.1 * i, 0.2); p2[i].q = 1.0; }
     fmm::detail::InitMathConstants(18);
     fmm::detail::cudaCopyMathConstants();
     try {
         fmm::gpu::FastMultipole3d<double> f(p, 1e-8, 8, 2);
         std::printf("%zu %zu\n", f.potentials.size(), f.forces.size());
+    } catch (const std::exception& e) { std::printf("error: %s\n", e.what()); }
+    try {
+        fmm::gpu::FastMultipole f2(p2, 1e-8, 8, 3);
+        std::printf("%zu %zu\n", f2.potentials.size(), f2.forces.size());
     } catch (const std::exception& e) { std::printf("error: %s\n", e.what()); }
     return 0;
 }
