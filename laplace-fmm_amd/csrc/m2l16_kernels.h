@@ -98,14 +98,16 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
     static_assert(256 % TC == 0, "loader split");
     __shared__ uint4 bs[2][CT][KC][2][64];
 
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, uniform
+    // row tiles w, w + 4, ... of this wave; all valid when RT divides over the 4 waves
+    constexpr bool kEven = (RT % 4) == 0;
     const int blk = xcd_local_block(blockIdx.x, gridDim.x);
-    // class-major block order: the workgroups resident on one XCD share a child
-    // class and direction part, so they stream the same operators (L2 reuse)
-    const int per_class = a.tiles * a.dsplit;
-    const int c = blk / per_class;
-    const int part = (blk - c * per_class) / a.tiles;
-    const int tile = blk - c * per_class - part * a.tiles;
+    // tile-major block order: the 8 child classes (and direction parts) of a
+    // tile are adjacent, so the workgroups of one XCD read overlapping sources
+    const int tile = blk / (8 * a.dsplit);
+    const int part = (blk - tile * 8 * a.dsplit) >> 3;
+    const int c = blk & 7;
     const int d_beg = part * kDirsPerClass / a.dsplit, d_end = (part + 1) * kDirsPerClass / a.dsplit;
     const int side = 1 << a.level;
 
@@ -205,19 +207,42 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
                     nal[r][kc] = ka[((rt * KC + kc) * 2 + 1) * 64];
                 }
         }
+        // keep the prefetch issued here, ahead of the MFMAs (not sunk to its use)
+        asm volatile("" ::: "memory");
+        // source fragments of K chunk kc + 1 are read from LDS while chunk kc's MFMAs run
+        uint4 fb[2][CT][2];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+            fb[0][ct][0] = bs[buf][ct][0][0][lane];
+            fb[0][ct][1] = bs[buf][ct][0][1][lane];
+        }
 #pragma unroll
         for (int kc = 0; kc < KC; ++kc) {
+            if (kc + 1 < KC) {
 #pragma unroll
-            for (int ct = 0; ct < CT; ++ct) {
-                const m16_h8 bh = __builtin_bit_cast(m16_h8, bs[buf][ct][kc][0][lane]);
-                const m16_h8 bl = __builtin_bit_cast(m16_h8, bs[buf][ct][kc][1][lane]);
+                for (int ct = 0; ct < CT; ++ct) {
+                    fb[(kc + 1) & 1][ct][0] = bs[buf][ct][kc + 1][0][lane];
+                    fb[(kc + 1) & 1][ct][1] = bs[buf][ct][kc + 1][1][lane];
+                }
+            }
 #pragma unroll
-                for (int r = 0; r < RW; ++r) {
-                    if (w + 4 * r < RT) {
-                        const m16_h8 xh = __builtin_bit_cast(m16_h8, ah[r][kc]);
-                        const m16_h8 xl = __builtin_bit_cast(m16_h8, al[r][kc]);
+            for (int r = 0; r < RW; ++r) {
+                if (kEven || w + 4 * r < RT) {
+                    const m16_h8 xh = __builtin_bit_cast(m16_h8, ah[r][kc]);
+                    const m16_h8 xl = __builtin_bit_cast(m16_h8, al[r][kc]);
+#pragma unroll
+                    for (int ct = 0; ct < CT; ++ct) {
+                        const m16_h8 bh = __builtin_bit_cast(m16_h8, fb[kc & 1][ct][0]);
                         acc[r][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bh, acc[r][ct], 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int ct = 0; ct < CT; ++ct) {
+                        const m16_h8 bl = __builtin_bit_cast(m16_h8, fb[kc & 1][ct][1]);
                         acc[r][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bl, acc[r][ct], 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int ct = 0; ct < CT; ++ct) {
+                        const m16_h8 bh = __builtin_bit_cast(m16_h8, fb[kc & 1][ct][0]);
                         acc[r][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xl, bh, acc[r][ct], 0, 0, 0);
                     }
                 }
@@ -249,7 +274,7 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
 #pragma unroll
         for (int r = 0; r < RW; ++r) {
             const int rt = w + 4 * r;
-            if (rt >= RT) continue;
+            if (!kEven && rt >= RT) continue;
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const int row = rt * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);

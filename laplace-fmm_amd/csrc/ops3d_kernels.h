@@ -238,15 +238,15 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
     constexpr int TW = rtr_tile(N);
     const bool lok = lane < TW;
     const int blk = xcd_local_block(blockIdx.x, gridDim.x);
-    // class-major block order: the workgroups resident on one XCD share a child
-    // class (and direction part), so they walk the same directions together
+    // tile-major block order: the 8 child classes (and direction parts) of a
+    // tile are adjacent, so the workgroups of one XCD read overlapping sources
     int tile, c, part = 0;
     if constexpr (KIND != KIND_M2M) {
-        const int per_class = L.tiles * L.dsplit;
-        c = blk / per_class;
-        const int r = blk - c * per_class;
-        part = r / L.tiles;
-        tile = r - part * L.tiles;
+        const int per_tile = 8 * L.dsplit;
+        tile = blk / per_tile;
+        const int r = blk - tile * per_tile;
+        part = r >> 3;
+        c = r & 7;
     } else {
         tile = blk;
         c = 0;
