@@ -341,31 +341,6 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     if (rc != LFMM_OK) return rc;
     std::vector<double2*> outer(depth, nullptr), inner(depth, nullptr);
     const bool far = depth >= 3;
-    if (far) {
-        for (int l = 2; l < depth; ++l) {
-            outer[l] = take<double2>(A, cnt[l] * NX);
-            inner[l] = take<double2>(A, cnt[l] * NX);
-            if (!outer[l] || !inner[l]) return LFMM_ERR_NOMEM;
-        }
-        P2MArgs pa{src, lbegin[leaf_level], lkeys[leaf_level], (int)cnt[leaf_level], leaf_level, geom, DT->knm,
-                   outer[leaf_level]};
-        LFMM_HIP(launch_p2m(N, pa, st));
-        for (int l = leaf_level - 1; l >= 2; --l) {
-            RtrLevel L{};
-            L.level = l;
-            L.num_parents = (int)cnt[l];
-            L.child_begin = lbegin[l];
-            L.child_keys = lkeys[l + 1];
-            L.src = outer[l + 1];
-            L.dst = outer[l];
-            LFMM_HIP(launch_rtr(N, 0, DT->rt, L, st));
-        }
-    }
-    double t_up = 0.0;
-    if (timed) {
-        LFMM_HIP(hipStreamSynchronize(st));
-        t_up = ms_since(T0) - t_tree;
-    }
     size_t nev = 0;
     int m2l_launches = 0, mixed_launches = 0;
     std::vector<char> ev_mixed;  // per event pair: matrix-core M2L or not
@@ -382,6 +357,34 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         xmax = take<uint32_t>(A, depth);
         if (!xp || !xmax) return LFMM_ERR_NOMEM;
         LFMM_HIP(fill_u32(xmax, depth, 0u, st));
+    }
+    if (far) {
+        for (int l = 2; l < depth; ++l) {
+            outer[l] = take<double2>(A, cnt[l] * NX);
+            inner[l] = take<double2>(A, cnt[l] * NX);
+            if (!outer[l] || !inner[l]) return LFMM_ERR_NOMEM;
+        }
+        P2MArgs pa{src, lbegin[leaf_level], lkeys[leaf_level], (int)cnt[leaf_level], leaf_level, geom, DT->knm,
+                   outer[leaf_level]};
+        // the matrix-core levels need their largest |coefficient|: the producers max-combine it
+        if (mixed && leaf_level >= kMixedFirstLevel) pa.absmax = xmax + leaf_level;
+        LFMM_HIP(launch_p2m(N, pa, st));
+        for (int l = leaf_level - 1; l >= 2; --l) {
+            RtrLevel L{};
+            L.level = l;
+            L.num_parents = (int)cnt[l];
+            L.child_begin = lbegin[l];
+            L.child_keys = lkeys[l + 1];
+            L.src = outer[l + 1];
+            L.dst = outer[l];
+            if (mixed && l >= kMixedFirstLevel) L.absmax = xmax + l;
+            LFMM_HIP(launch_rtr(N, 0, DT->rt, L, st));
+        }
+    }
+    double t_up = 0.0;
+    if (timed) {
+        LFMM_HIP(hipStreamSynchronize(st));
+        t_up = ms_since(T0) - t_tree;
     }
     // direction parts: levels with fewer workgroups than the chip holds split
     // their 189 directions over more workgroups (deterministic partial sums)
@@ -425,7 +428,6 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                     if (!m.partial) return LFMM_ERR_NOMEM;
                     LFMM_HIP(fill_u32((uint32_t*)m.partial, (int64_t)m.dsplit * cnt[l] * NX * 4, 0u, st));
                 }
-                LFMM_HIP(launch_absmax((const double*)outer[l], cnt[l] * NX * 2, xmax + l, st));
                 LFMM_HIP(launch_m2l16_pack(N, m, st));
                 LFMM_HIP(launch_rtr(N, 2, DT->rt, L, st));
                 if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
@@ -475,6 +477,12 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     la.local = far ? inner[leaf_level] : nullptr;
     la.knm = DT->knm;
     la.perm = order;
+    {
+        int2* nbr = take<int2>(A, cnt[leaf_level] * 27);
+        if (!nbr) return LFMM_ERR_NOMEM;
+        LFMM_HIP(launch_leaf_ranges(lkeys[leaf_level], lbegin[leaf_level], (int)cnt[leaf_level], leaf_level, la.lut, nbr, st));
+        la.nbr = nbr;
+    }
     la.n = n;
     la.n_out = n_out;
     la.exclude_self = (flags & LFMM_FLAG_EXCLUDE_SELF) ? 1 : 0;

@@ -98,6 +98,14 @@ __device__ inline double2 cmulc(double2 a, double2 b)  // a * conj(b)
     return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 
+// *out = max(*out, m) over the wave, one atomic per wave (m >= 0; non-negative
+// floats order like their bit patterns).  Call from every lane of the wave.
+__device__ inline void wave_absmax(double m, uint32_t* out)
+{
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint((float)m));
+}
+
 // Find box index by key at a level: dense table when present, otherwise a
 // binary search over the sorted unique keys.  Returns -1 when absent.
 __device__ inline int find_box(uint64_t key, const int* __restrict__ lut, const uint64_t* __restrict__ keys, int size)

@@ -14,6 +14,7 @@ struct P2MArgs {
     const double* geom;        // {shift x, y, z, scale} (device)
     const double* knm;         // K_nm table, tri(N+1)
     double2* outer;            // normalised multipoles of the leaves
+    uint32_t* absmax = nullptr;   // if set: max-combined largest |coefficient| (float bits)
 };
 
 struct RtrTables {
@@ -51,6 +52,7 @@ struct RtrLevel {
     int tiles = 0;
     int dsplit = 1;
     double2* partial = nullptr;
+    uint32_t* absmax = nullptr;   // M2M: if set, max-combined largest |coefficient| written (float bits)
 };
 
 struct LeafArgs {
@@ -65,6 +67,7 @@ struct LeafArgs {
     const double2* local;      // normalised locals of the leaves
     const double* knm;
     const uint32_t* perm;      // sorted -> input index
+    const int2* nbr;           // [nleaf][27] neighbour particle ranges (launch_leaf_ranges)
     int64_t n;
     int64_t n_out;             // outputs for input indices < n_out only (targets first)
     int exclude_self;
@@ -101,11 +104,13 @@ constexpr int m16_targets(int N) { return N <= 16 ? 64 : 32; }
 hipError_t launch_p2m(int N, const P2MArgs& a, hipStream_t st);
 hipError_t launch_rtr(int N, int kind, const RtrTables& T, const RtrLevel& L, hipStream_t st);
 hipError_t launch_leaf(int N, bool local, const LeafArgs& a, hipStream_t st);
+// nbr[leaf][27] = particle ranges of the 27 neighbour leaves (tree.hip)
+hipError_t launch_leaf_ranges(const uint64_t* leaf_key, const int32_t* leaf_begin, int nleaf, int level,
+                              const int32_t* lut, int2* nbr, hipStream_t st);
 hipError_t launch_m2l16_pack(int N, const M2L16Args& a, hipStream_t st);
 hipError_t launch_m2l16(int N, const M2L16Args& a, hipStream_t st);
 // dst[i] = (add ? dst[i] : 0) + sum_p partial[p * count + i]   (m2l16.hip)
 hipError_t launch_sum_parts(const double2* partial, int parts, int64_t count, double2* dst, bool add, hipStream_t st);
-// *out = max(*out, largest |v[i]|) as float bits (m2l16.hip)
-hipError_t launch_absmax(const double* v, int64_t count, uint32_t* out, hipStream_t st);
+
 
 }  // namespace lfmm

@@ -71,39 +71,88 @@ struct Partition {
 // with the Cartesian recurrences
 //   S_m^m = -(2m-1)(ux + i uy) S_{m-1}^{m-1},  S_{m+1}^m = (2m+1) uz S_m^m,
 //   (n-m) S_n^m = (2n-1) uz S_{n-1}^m - (n+m-1) |u|^2 S_{n-2}^m.
-// A wave handles one leaf: 16 particle slots x 4 coefficient groups; the 16
-// partial sums of a group are reduced with DPP inside the 16-lane row.
+// A wave handles 4 leaves, one 16-lane row per leaf and one particle per
+// lane (rounds of 16 particles).  Harmonics are produced column by column
+// (m-major position pos); each lane writes q conj(S) of 16 consecutive
+// positions into a wave-private LDS tile, then lane (row, slot) sums entry
+// `slot` over the 16 lanes of its row.  Lane (row, slot) thus accumulates
+// positions slot, slot + 16, ... of its leaf; K_nm scales the final sums.
+template <int N>
+constexpr int p2m_pos(int n, int m) { return m * N - m * (m - 1) / 2 + (n - m); }
+
+template <int N>
+struct P2MOrder {  // position -> tri index
+    int idx[tri(kMaxTerms3d)];
+    constexpr P2MOrder() : idx()
+    {
+        for (int m = 0; m < N; ++m)
+            for (int n = m; n < N; ++n) idx[p2m_pos<N>(n, m)] = tri(n) + m;
+    }
+};
+
 template <int N>
 __global__ void __launch_bounds__(256) k_p2m(P2MArgs a, const double* __restrict__ knm)
 {
     constexpr int NX = tri(N);
-    constexpr int CH = (NX + 3) / 4;
-    const int lane = threadIdx.x & 63;
-    const int leaf = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (leaf >= a.nleaf) return;
-    const int slot = lane & 15, grp = lane >> 4;
-    const uint64_t key = a.leaf_key[leaf];
-    const double scale = a.geom[3];
-    const double cx = box_centre(compact3(key >> 2), a.level, scale, a.geom[0]);
-    const double cy = box_centre(compact3(key >> 1), a.level, scale, a.geom[1]);
-    const double cz = box_centre(compact3(key), a.level, scale, a.geom[2]);
-    const double ih = ldexp(1.0, a.level) / scale;
-    double2 acc[CH];
+    constexpr int NCH = (NX + 15) / 16;
+    constexpr P2MOrder<N> ORD{};
+    __shared__ double2 red[4][16][65];  // per wave [entry][lane], rows padded: conflict-free column reads
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int row = lane >> 4, slot = lane & 15;
+    const int leaf = (blockIdx.x * 4 + w) * 4 + row;
+    const bool ok = leaf < a.nleaf;
+    double cx = 0.0, cy = 0.0, cz = 0.0, ih = 0.0;
+    int b = 0, e = 0;
+    if (ok) {
+        const uint64_t key = a.leaf_key[leaf];
+        const double scale = a.geom[3];
+        cx = box_centre(compact3(key >> 2), a.level, scale, a.geom[0]);
+        cy = box_centre(compact3(key >> 1), a.level, scale, a.geom[1]);
+        cz = box_centre(compact3(key), a.level, scale, a.geom[2]);
+        ih = ldexp(1.0, a.level) / scale;
+        b = a.leaf_begin[leaf];
+        e = a.leaf_begin[leaf + 1];
+    }
+    int rounds = (e - b + 15) >> 4;
+    for (int off = 32; off > 0; off >>= 1) rounds = max(rounds, __shfl_xor(rounds, off));
+    double2 (*R)[65] = red[w];
+    double2 acc[NCH];
 #pragma unroll
-    for (int k = 0; k < CH; ++k) acc[k] = make_double2(0.0, 0.0);
-    const int b = a.leaf_begin[leaf], e = a.leaf_begin[leaf + 1];
-    for (int p = b + slot; p < e; p += 16) {
-        const double4 s = a.src[p];
-        const double ux = (s.x - cx) * ih, uy = (s.y - cy) * ih, uz = (s.z - cz) * ih, q = s.w;
+    for (int c = 0; c < NCH; ++c) acc[c] = make_double2(0.0, 0.0);
+    for (int rd = 0; rd < rounds; ++rd) {
+        const int p = b + slot + 16 * rd;
+        double ux = 0.0, uy = 0.0, uz = 0.0, q = 0.0;
+        if (p < e) {
+            const double4 s4 = a.src[p];
+            ux = (s4.x - cx) * ih;
+            uy = (s4.y - cy) * ih;
+            uz = (s4.z - cz) * ih;
+            q = s4.w;
+        }
         const double r2 = ux * ux + uy * uy + uz * uz;
-        auto put = [&](auto I, double2 S) {
-            constexpr int idx = decltype(I)::value;
-            constexpr int g = idx / CH, k = idx % CH;
-            const double f = q * knm[idx];
-            const double vr = (grp == g) ? f * S.x : 0.0;
-            const double vi = (grp == g) ? -f * S.y : 0.0;
-            acc[k].x += vr;
-            acc[k].y += vi;
+        auto put = [&](auto PI, double2 S) {
+            constexpr int pos = decltype(PI)::value;
+            R[pos & 15][lane] = make_double2(q * S.x, -q * S.y);  // K_nm is applied to the sums
+            // keep harmonics from being computed ahead of their stores (register pressure)
+            __builtin_amdgcn_sched_barrier(0);
+            if constexpr ((pos & 15) == 15 || pos == NX - 1) {
+                // the chunk is complete: lane (row, slot) sums entry `slot` of its row
+                asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+                double sx = 0.0, sy = 0.0;
+                if (slot <= (pos & 15)) {
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const double2 v = R[slot][row * 16 + j];
+                        sx += v.x;
+                        sy += v.y;
+                    }
+                }
+                // materialise the sums here (the loaded entries die now, not at the loop end)
+                asm volatile("" : "+v"(sx), "+v"(sy));
+                acc[pos >> 4].x += sx;
+                acc[pos >> 4].y += sy;
+                asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            }
         };
         double2 smm = make_double2(1.0, 0.0);
         sfor<0, N>([&](auto M) {
@@ -112,34 +161,38 @@ __global__ void __launch_bounds__(256) k_p2m(P2MArgs a, const double* __restrict
                 const double c = -(double)(2 * m - 1);
                 smm = make_double2(c * (ux * smm.x - uy * smm.y), c * (ux * smm.y + uy * smm.x));
             }
-            put(std::integral_constant<int, tri(m) + m>{}, smm);
+            put(std::integral_constant<int, p2m_pos<N>(m, m)>{}, smm);
             if constexpr (m + 1 < N) {
                 double2 s2 = smm;
                 double2 s1 = make_double2((2 * m + 1) * uz * smm.x, (2 * m + 1) * uz * smm.y);
-                put(std::integral_constant<int, tri(m + 1) + m>{}, s1);
+                put(std::integral_constant<int, p2m_pos<N>(m + 1, m)>{}, s1);
                 sfor<m + 2, N>([&](auto NN) {
                     constexpr int n = decltype(NN)::value;
                     constexpr double c1 = (double)(2 * n - 1) / (double)(n - m);
                     constexpr double c2 = (double)(n + m - 1) / (double)(n - m);
                     const double2 s0 = make_double2(c1 * uz * s1.x - c2 * r2 * s2.x, c1 * uz * s1.y - c2 * r2 * s2.y);
-                    put(std::integral_constant<int, tri(n) + m>{}, s0);
+                    put(std::integral_constant<int, p2m_pos<N>(n, m)>{}, s0);
                     s2 = s1;
                     s1 = s0;
                 });
             }
         });
     }
+    double mx = 0.0;
+    if (ok) {
+        double2* out = a.outer + (int64_t)leaf * NX;
 #pragma unroll
-    for (int k = 0; k < CH; ++k) {
-        acc[k].x = row16_sum(acc[k].x);
-        acc[k].y = row16_sum(acc[k].y);
+        for (int c = 0; c < NCH; ++c) {
+            const int pos = c * 16 + slot;
+            if (pos < NX) {
+                const double k = knm[ORD.idx[pos]];
+                const double2 v = make_double2(k * acc[c].x, k * acc[c].y);
+                out[ORD.idx[pos]] = v;
+                mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+            }
+        }
     }
-    double2* out = a.outer + (int64_t)leaf * NX;
-#pragma unroll
-    for (int k = 0; k < CH; ++k) {
-        const int idx = grp * CH + k;
-        if (slot == (k & 15) && idx < NX) out[idx] = acc[k];
-    }
+    if (a.absmax) wave_absmax(mx, a.absmax);
 }
 
 // ===================================================================== rotate-translate-rotate
@@ -413,6 +466,7 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
             lds_barrier();
         }
     }
+    double mx = 0.0;
     if (tgt >= 0) {
         double2* out = (L.dsplit > 1 ? L.partial + (int64_t)part * L.size * NX : L.dst) + (int64_t)tgt * NX;
         sfor<0, N>([&](auto JJ) {
@@ -421,10 +475,13 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
                 sfor<0, j + 1>([&](auto MU) {
                     constexpr int mu = decltype(MU)::value;
                     out[tri(j) + mu] = acc[tri(j) + mu];
+                    if constexpr (KIND == KIND_M2M) mx = fmax(mx, fmax(fabs(acc[tri(j) + mu].x), fabs(acc[tri(j) + mu].y)));
                 });
             }
         });
     }
+    if constexpr (KIND == KIND_M2M)
+        if (L.absmax) wave_absmax(mx, L.absmax);
 }
 
 template <int N, int KIND>
@@ -549,7 +606,6 @@ __global__ void __launch_bounds__(256) k_leaf(LeafArgs a, const double2* __restr
     const uint64_t key = a.leaf_key[leaf];
     const uint32_t ix = compact3(key >> 2), iy = compact3(key >> 1), iz = compact3(key);
     const double4 t = a.src[s];
-    const int side = 1 << a.level;
     // far field first: its registers are dead before the near-field loop starts
     double ph = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
     if constexpr (LOCAL) {
@@ -566,26 +622,34 @@ __global__ void __launch_bounds__(256) k_leaf(LeafArgs a, const double2* __restr
         fy = -gy * ih2;
         fz = -gz * ih2;
     }
-    for (int o = 0; o < 27; ++o) {
-        const int nx = (int)ix + o / 9 - 1, ny = (int)iy + (o / 3) % 3 - 1, nz = (int)iz + o % 3 - 1;
-        if (nx < 0 || ny < 0 || nz < 0 || nx >= side || ny >= side || nz >= side) continue;
-        const int nb = (o == 13) ? leaf : find_box(morton3(nx, ny, nz), a.lut, a.leaf_key, a.nleaf);
-        if (nb < 0) continue;
-        const int b = a.leaf_begin[nb], e = a.leaf_begin[nb + 1];
-        for (int j = b; j < e; ++j) {
-            const double4 p = a.src[j];
-            if (a.exclude_self && j == (int)s) continue;
-            const double dx = t.x - p.x, dy = t.y - p.y, dzz = t.z - p.z;
-            const double r2 = fmax(fma(dx, dx, fma(dy, dy, dzz * dzz)), 1.e-8);
-            const double ir = rsqrt_fp64(r2);
-            const double qi = p.w * ir;
-            const double qi3 = qi * (ir * ir);
-            ph += qi;
-            fx = fma(dx, qi3, fx);
-            fy = fma(dy, qi3, fy);
-            fz = fma(dzz, qi3, fz);
+    // near field over the 27 neighbour leaves (ranges from k_leaf_ranges), four
+    // sources per step with their loads issued together, then the remainder
+    auto pair = [&](const double4& p) {
+        const double dx = t.x - p.x, dy = t.y - p.y, dzz = t.z - p.z;
+        const double r2 = fmax(fma(dx, dx, fma(dy, dy, dzz * dzz)), 1.e-8);
+        const double ir = rsqrt_fp64(r2);
+        const double qi = p.w * ir;
+        const double qi3 = qi * (ir * ir);
+        ph += qi;
+        fx = fma(dx, qi3, fx);
+        fy = fma(dy, qi3, fy);
+        fz = fma(dzz, qi3, fz);
+    };
+    const int2* __restrict__ nr = a.nbr + (int64_t)leaf * 27;
+    for (int k = 0; k < 27; ++k) {
+        const int2 r = nr[k];
+        int j = r.x;
+        for (; j + 3 < r.y; j += 4) {
+            const double4 p0 = a.src[j], p1 = a.src[j + 1], p2 = a.src[j + 2], p3 = a.src[j + 3];
+            pair(p0);
+            pair(p1);
+            pair(p2);
+            pair(p3);
         }
+        for (; j < r.y; ++j) pair(a.src[j]);
     }
+    // the self pair (clamped at 1e-4) was included; its field term is exactly zero
+    if (a.exclude_self) ph -= t.w * rsqrt_fp64(1.e-8);
     if (a.pot) a.pot[o] = ph;
     if (a.field) {
         a.field[3 * o] = fx;
@@ -598,7 +662,7 @@ __global__ void __launch_bounds__(256) k_leaf(LeafArgs a, const double2* __restr
 template <int N>
 hipError_t do_p2m(const P2MArgs& a, hipStream_t st)
 {
-    const int nb = (a.nleaf + 3) / 4;
+    const int nb = (a.nleaf + 15) / 16;  // 4 waves x 4 leaves
     if (nb > 0) hipLaunchKernelGGL(k_p2m<N>, dim3(nb), dim3(256), 0, st, a, a.knm);
     return hipGetLastError();
 }

@@ -14,6 +14,7 @@
 // key lookup in the operator kernels.
 #include "device_common.h"
 #include "tree.h"
+#include "ops3d.h"
 
 namespace lfmm {
 
@@ -506,6 +507,26 @@ __global__ void __launch_bounds__(256) k_count_p2p(const uint64_t* __restrict__ 
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
 }
 
+// ------------------------------------------------------------------ near-field ranges
+__global__ void __launch_bounds__(256) k_leaf_ranges(const uint64_t* __restrict__ leaf_key,
+                                                     const int32_t* __restrict__ leaf_begin, int nleaf, int level,
+                                                     const int32_t* __restrict__ lut, int2* __restrict__ nbr)
+{
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= (int64_t)nleaf * 27) return;
+    const int leaf = (int)(g / 27), o = (int)(g % 27);
+    const uint64_t k = leaf_key[leaf];
+    const int x = (int)compact3(k >> 2) + o / 9 - 1, y = (int)compact3(k >> 1) + (o / 3) % 3 - 1,
+              z = (int)compact3(k) + o % 3 - 1;
+    const int side = 1 << level;
+    int2 r = make_int2(0, 0);
+    if (x >= 0 && y >= 0 && z >= 0 && x < side && y < side && z < side) {
+        const int j = (o == 13) ? leaf : lookup(morton3(x, y, z), lut, leaf_key, nleaf);
+        if (j >= 0) r = make_int2(leaf_begin[j], leaf_begin[j + 1]);
+    }
+    nbr[g] = r;
+}
+
 // ================================================================== host side
 static inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
@@ -693,6 +714,14 @@ hipError_t count_ops(int dim, const std::vector<uint64_t*>& keys, const std::vec
     *m2l_ops = total;
     *p2p_pairs = (int64_t)h[1];
     return e;
+}
+
+hipError_t launch_leaf_ranges(const uint64_t* leaf_key, const int32_t* leaf_begin, int nleaf, int level,
+                              const int32_t* lut, int2* nbr, hipStream_t st)
+{
+    const int64_t n = (int64_t)nleaf * 27;
+    if (n > 0) hipLaunchKernelGGL(k_leaf_ranges, dim3(nblk(n, 256)), dim3(256), 0, st, leaf_key, leaf_begin, nleaf, level, lut, nbr);
+    return hipGetLastError();
 }
 
 hipError_t fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st)
