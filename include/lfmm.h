@@ -193,9 +193,10 @@ int64_t lfmm_host_op_tables(int N, int which, void* out, int64_t capacity);
  * (0..315, the order of table 5) for N terms, N^2 x N^2 row-major doubles.
  * lfmm_host_m2l16_pack: its split-fp16 packing for all directions as the
  * matrix-core kernel reads it (LFMM_FLAG_MIXED); returns the element count
- * (uint16 bit patterns) and the scale *kinv, or a negative status. */
+ * (uint16 bit patterns), the scale *kinv and, if kpre is not NULL, the block
+ * prefixes [316][ceil(N^2/32)][2] = (k3, k1), or a negative status. */
 int lfmm_host_m2l_dense(int N, int dir, double* out);
-int64_t lfmm_host_m2l16_pack(int N, uint16_t* out, int64_t capacity, double* kinv);
+int64_t lfmm_host_m2l16_pack(int N, uint16_t* out, int64_t capacity, double* kinv, int32_t* kpre);
 
 /* Library version string. */
 const char* lfmm_version(void);

@@ -115,7 +115,8 @@ def lib():
         L.lfmm_host_op_tables.restype = ctypes.c_int64
         L.lfmm_host_m2l_dense.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         L.lfmm_host_m2l_dense.restype = ctypes.c_int
-        L.lfmm_host_m2l16_pack.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+        L.lfmm_host_m2l16_pack.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                           ctypes.c_void_p]
         L.lfmm_host_m2l16_pack.restype = ctypes.c_int64
         _lib = L
     return _lib
@@ -317,15 +318,18 @@ def host_m2l_dense(N, direction):
 
 
 def host_m2l16_pack(N):
-    """Split-fp16 operator fragments of all directions: (uint16 array, kinv)."""
+    """Split-fp16 operator fragments of all directions: (uint16 array, kinv,
+    block prefixes int32 (316, ceil(N^2/32), 2) = (k3, k1))."""
     kinv = ctypes.c_double(0.0)
-    count = lib().lfmm_host_m2l16_pack(int(N), None, 0, ctypes.byref(kinv))
+    count = lib().lfmm_host_m2l16_pack(int(N), None, 0, ctypes.byref(kinv), None)
     if count < 0:
         raise LfmmError(int(-count), "lfmm_host_m2l16_pack")
     out = np.zeros(count, dtype=np.uint16)
-    if lib().lfmm_host_m2l16_pack(int(N), out.ctypes.data, count, ctypes.byref(kinv)) != count:
+    rt = (N * N + 31) // 32
+    kpre = np.zeros((316, rt, 2), dtype=np.int32)
+    if lib().lfmm_host_m2l16_pack(int(N), out.ctypes.data, count, ctypes.byref(kinv), kpre.ctypes.data) != count:
         raise LfmmError(1, "lfmm_host_m2l16_pack")
-    return out, kinv.value
+    return out, kinv.value, kpre
 
 
 class FastMultipole3d:

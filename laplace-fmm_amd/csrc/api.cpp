@@ -80,15 +80,19 @@ int get_m16(lfmm_ctx* ctx, int N, const DevTables** out)
         OpTables T;
         build_op_tables(N, T);
         std::vector<uint16_t> pk;
+        std::vector<int> kp;
         double kinv = 0.0;
-        pack_m2l16(T, pk, kinv);
-        uint4* p = nullptr;
-        if (hipMalloc(&p, pk.size() * 2) != hipSuccess) return LFMM_ERR_NOMEM;
-        if (hipMemcpy(p, pk.data(), pk.size() * 2, hipMemcpyHostToDevice) != hipSuccess) {
+        pack_m2l16(T, pk, kinv, kp);
+        const size_t bk = (pk.size() * 2 + 255) & ~(size_t)255;
+        char* p = nullptr;
+        if (hipMalloc(&p, bk + kp.size() * 4) != hipSuccess) return LFMM_ERR_NOMEM;
+        if (hipMemcpy(p, pk.data(), pk.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p + bk, kp.data(), kp.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
             (void)hipFree(p);
             return LFMM_ERR_HIP;
         }
-        D.m16 = p;
+        D.m16 = (uint4*)p;
+        D.m16_kpre = (const int2*)(p + bk);
         D.m16_kinv = kinv;
     }
     *out = &D;
@@ -407,6 +411,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                 // FP64 L2L into the level, then += the matrix-core M2L
                 M2L16Args m{};
                 m.kmat = DT->m16;
+                m.kpre = DT->m16_kpre;
                 m.outer = outer[l];
                 m.xp = xp;
                 m.xmax = xmax + l;
@@ -773,16 +778,18 @@ int lfmm_host_m2l_dense(int N, int dir, double* out)
     }
 }
 
-int64_t lfmm_host_m2l16_pack(int N, uint16_t* out, int64_t capacity, double* kinv)
+int64_t lfmm_host_m2l16_pack(int N, uint16_t* out, int64_t capacity, double* kinv, int32_t* kpre)
 {
     if (N < 1 || N > LFMM_MAX_TERMS_3D) return -(int64_t)LFMM_ERR_TERMS;
     try {
         OpTables T;
         build_op_tables(N, T);
         std::vector<uint16_t> pk;
+        std::vector<int> kp;
         double ki = 0.0;
-        pack_m2l16(T, pk, ki);
+        pack_m2l16(T, pk, ki, kp);
         if (kinv) *kinv = ki;
+        if (kpre) std::memcpy(kpre, kp.data(), kp.size() * sizeof(int32_t));
         if (out && capacity >= (int64_t)pk.size()) std::memcpy(out, pk.data(), pk.size() * 2);
         return (int64_t)pk.size();
     } catch (...) {

@@ -79,6 +79,7 @@ struct DevTables {
     RtrTables rt{};
     const double* knm = nullptr;
     uint4* m16 = nullptr;      // split-fp16 dense M2L operators (built on first mixed solve)
+    const int2* m16_kpre = nullptr;  // their block prefixes (same allocation)
     double m16_kinv = 0.0;
 };
 

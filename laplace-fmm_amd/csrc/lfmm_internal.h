@@ -70,8 +70,10 @@ void build_op_tables_2d(int N, OpTables2d& T);
 int m2l_real_index(int n, int m, int part);
 void m2l_dense_matrix(const OpTables& T, int dir, double* K);
 // All M2L operators scaled by 1/kinv (a power of two) and split into fp16
-// hi/lo A-operand fragments of the matrix-core M2L kernel.
-void pack_m2l16(const OpTables& T, std::vector<uint16_t>& out, double& kinv);
+// hi/lo A-operand fragments of the matrix-core M2L kernel, and per
+// (direction, row tile of 32) the block prefixes (k3, k1): K chunks below k3
+// need all three split products, below k1 the hi*hi product alone.
+void pack_m2l16(const OpTables& T, std::vector<uint16_t>& out, double& kinv, std::vector<int>& kpre);
 
 // Small host helpers exposed for tests through the C ABI debug entry points.
 double host_knm(int n, int m);

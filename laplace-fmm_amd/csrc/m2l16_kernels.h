@@ -12,8 +12,12 @@
 // split into fp16 hi + lo (22 significant bits).  Each product is
 //   hi*hi + hi*lo + lo*hi   (the lo*lo term, ~2^-22 relative, is dropped)
 // computed exactly on the matrix cores and accumulated in FP32 over all K and
-// all 189 directions.  The result is scaled back and added in FP64 to the
-// local expansion, which already holds the FP64 L2L of the parent.
+// all 189 directions.  From 12 terms, operator blocks (32 rows x 16 K) whose
+// largest entry is below 2^-11 of the largest operator entry use hi*hi alone,
+// and blocks below 2^-22 are skipped: either way the dropped terms are bounded
+// by 2^-22 max|K| max|X|, the absolute bound of the split itself.  The result
+// is scaled back and added in FP64 to the local expansion, which already holds
+// the FP64 L2L of the parent.
 //
 // Operands (32x32x16 f16 MFMA; lane l, slot s in 0..7):
 //   A  row l % 32, K index 8 * (l / 32) + s     (operator, from global/L2)
@@ -86,9 +90,14 @@ __global__ void __launch_bounds__(256) k_m2l16_pack(const double2* __restrict__ 
     dst[2 + half] = __builtin_bit_cast(uint4, lv);
 }
 
+// Block skipping (the prefix table) pays where the operator blocks are many
+// and sparse; below 12 terms the straight-line MFMA stream is faster.
+constexpr bool m16_skip(int N) { return N >= 12; }
+
 template <int N, int CT>
 __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restrict__ kmat, const uint4* __restrict__ xp,
-                                               const int* __restrict__ class_dirs, const int* __restrict__ dir_vec)
+                                               const int* __restrict__ class_dirs, const int* __restrict__ dir_vec,
+                                               const int2* __restrict__ kpre)
 {
     using S = M16<N>;
     constexpr int RT = S::RT, KC = S::KC, RW = S::RW, NR = S::NR, PIECES = S::PIECES;
@@ -152,9 +161,28 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
     };
     constexpr bool kFullPieces = (PIECES % TPC) == 0;
     auto piece_ok = [&](int i) { return kFullPieces || lq + TPC * i < PIECES; };
-    // operator fragments of this wave's row tiles for direction step d
-    auto op_base = [&](int d) -> const uint4* {
-        return kmat + (int64_t)class_dirs[c * kDirsPerClass + d] * (RT * KC * 2 * 64) + lane;
+    // Operator fragments of this wave's row tiles stream through a rolling
+    // window: chunk kc + LA is loaded while chunk kc is multiplied (across the
+    // step boundary into the next direction).  Only the blocks the precision
+    // needs are loaded and multiplied: per (direction, row tile), chunks
+    // kc < k3 use hi*hi + hi*lo + lo*hi, k3 <= kc < k1 hi*hi alone, the rest
+    // nothing (host table kpre; see pack_m2l16).
+    // one row tile per wave: the whole next step is prefetched; two: 4 chunks ahead
+    constexpr int LA = RW == 1 ? KC : 4;
+    auto dir_of = [&](int d) { return class_dirs[c * kDirsPerClass + d]; };
+    auto row_tile = [&](int r) { return kEven ? w + 4 * r : min(w + 4 * r, RT - 1); };
+    auto prefix = [&](int dir, int r) -> int2 {
+        if (!kEven && w + 4 * r >= RT) return make_int2(0, 0);
+        if constexpr (!m16_skip(N)) return make_int2(KC, KC);
+        const int2 v = kpre[dir * RT + row_tile(r)];
+        // wave-uniform: keep the block bounds in scalar registers (scalar branches)
+        return make_int2(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y));
+    };
+    // unneeded fragments are read from one fixed block (cache-resident), so
+    // every load is unconditional and the loop stays straight-line code
+    auto frag = [&](int dir, int r, int kc, int hl, bool need) -> uint4 {
+        const int64_t blk = need ? (int64_t)dir * (RT * KC * 2) + (row_tile(r) * KC + kc) * 2 + hl : 0;
+        return kmat[blk * 64 + lane];
     };
 
     m16_acc acc[RW][CT];
@@ -165,7 +193,19 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[r][ct][v] = 0.0f;
 
-    uint4 ah[RW][KC], al[RW][KC];
+    // step d_beg: sources staged, first LA operator chunks loaded
+    int dcur = dir_of(d_beg);
+    int2 pc[RW];
+    uint4 hh[RW][LA], hl[RW][LA];
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+        pc[r] = prefix(dcur, r);
+#pragma unroll
+        for (int k = 0; k < LA; ++k) {
+            hh[r][k] = frag(dcur, r, k, 0, k < pc[r].y);
+            hl[r][k] = frag(dcur, r, k, 1, k < pc[r].x);
+        }
+    }
     {
         const uint4* p = source_piece(d_beg);
         uint4 t[PPT];
@@ -174,42 +214,24 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
 #pragma unroll
         for (int i = 0; i < PPT; ++i)
             if (piece_ok(i)) *stage_slot(0, i) = t[i];
-        const uint4* ka = op_base(d_beg);
-#pragma unroll
-        for (int r = 0; r < RW; ++r)
-#pragma unroll
-            for (int kc = 0; kc < KC; ++kc) {
-                const int rt = min(w + 4 * r, RT - 1);
-                ah[r][kc] = ka[((rt * KC + kc) * 2) * 64];
-                al[r][kc] = ka[((rt * KC + kc) * 2 + 1) * 64];
-            }
     }
     lds_barrier();
     for (int d = d_beg; d < d_end; ++d) {
         const int buf = (d - d_beg) & 1;
         const int dn = min(d + 1, d_end - 1);  // the last step re-stages its own data (never read)
-        // next step's sources and operator fragments, in flight under the MFMAs
+        const int dnext = dir_of(dn);
+        int2 pn[RW];
+#pragma unroll
+        for (int r = 0; r < RW; ++r) pn[r] = prefix(dnext, r);
+        // next step's sources, in flight under the MFMAs
         uint4 nb[PPT];
         {
             const uint4* p = source_piece(dn);
 #pragma unroll
             for (int i = 0; i < PPT; ++i) nb[i] = piece_ok(i) ? p[TPC * i] : make_uint4(0, 0, 0, 0);
         }
-        uint4 nah[RW][KC], nal[RW][KC];
-        {
-            const uint4* ka = op_base(dn);
-#pragma unroll
-            for (int r = 0; r < RW; ++r)
-#pragma unroll
-                for (int kc = 0; kc < KC; ++kc) {
-                    const int rt = min(w + 4 * r, RT - 1);
-                    nah[r][kc] = ka[((rt * KC + kc) * 2) * 64];
-                    nal[r][kc] = ka[((rt * KC + kc) * 2 + 1) * 64];
-                }
-        }
-        // keep the prefetch issued here, ahead of the MFMAs (not sunk to its use)
-        asm volatile("" ::: "memory");
-        // source fragments of K chunk kc + 1 are read from LDS while chunk kc's MFMAs run
+        uint4 ch[RW][KC], cl[RW][KC];  // this step's chunks LA.. (loaded LA chunks ahead)
+        uint4 nh[RW][LA], nl[RW][LA];  // next step's chunks 0..LA-1
         uint4 fb[2][CT][2];
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) {
@@ -218,6 +240,19 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
         }
 #pragma unroll
         for (int kc = 0; kc < KC; ++kc) {
+            // issue the operator chunk LA ahead
+            const int ka = kc + LA;
+#pragma unroll
+            for (int r = 0; r < RW; ++r) {
+                if (ka < KC) {
+                    ch[r][ka] = frag(dcur, r, ka, 0, ka < pc[r].y);
+                    cl[r][ka] = frag(dcur, r, ka, 1, ka < pc[r].x);
+                } else {
+                    nh[r][ka - KC] = frag(dnext, r, ka - KC, 0, ka - KC < pn[r].y);
+                    nl[r][ka - KC] = frag(dnext, r, ka - KC, 1, ka - KC < pn[r].x);
+                }
+            }
+            // source fragments of chunk kc + 1 are read while chunk kc multiplies
             if (kc + 1 < KC) {
 #pragma unroll
                 for (int ct = 0; ct < CT; ++ct) {
@@ -227,14 +262,18 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
             }
 #pragma unroll
             for (int r = 0; r < RW; ++r) {
-                if (kEven || w + 4 * r < RT) {
-                    const m16_h8 xh = __builtin_bit_cast(m16_h8, ah[r][kc]);
-                    const m16_h8 xl = __builtin_bit_cast(m16_h8, al[r][kc]);
+                const uint4 ah = kc < LA ? hh[r][kc] : ch[r][kc];
+                const uint4 al = kc < LA ? hl[r][kc] : cl[r][kc];
+                const m16_h8 xh = __builtin_bit_cast(m16_h8, ah);
+                const m16_h8 xl = __builtin_bit_cast(m16_h8, al);
+                if (kc < pc[r].y) {
 #pragma unroll
                     for (int ct = 0; ct < CT; ++ct) {
                         const m16_h8 bh = __builtin_bit_cast(m16_h8, fb[kc & 1][ct][0]);
                         acc[r][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bh, acc[r][ct], 0, 0, 0);
                     }
+                }
+                if (kc < pc[r].x) {
 #pragma unroll
                     for (int ct = 0; ct < CT; ++ct) {
                         const m16_h8 bl = __builtin_bit_cast(m16_h8, fb[kc & 1][ct][1]);
@@ -252,12 +291,15 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
         for (int i = 0; i < PPT; ++i)
             if (piece_ok(i)) *stage_slot(buf ^ 1, i) = nb[i];
 #pragma unroll
-        for (int r = 0; r < RW; ++r)
+        for (int r = 0; r < RW; ++r) {
+            pc[r] = pn[r];
 #pragma unroll
-            for (int kc = 0; kc < KC; ++kc) {
-                ah[r][kc] = nah[r][kc];
-                al[r][kc] = nal[r][kc];
+            for (int k = 0; k < LA; ++k) {
+                hh[r][k] = nh[r][k];
+                hl[r][k] = nl[r][k];
             }
+        }
+        dcur = dnext;
         lds_barrier();
     }
 
@@ -310,7 +352,7 @@ hipError_t do_m2l16(const M2L16Args& a, hipStream_t st)
     if (tiles != a.tiles || a.dsplit < 1 || a.dsplit > kDirsPerClass) return hipErrorInvalidValue;
     if (tiles > 0)
         hipLaunchKernelGGL((k_m2l16<N, m16_ct(N)>), dim3(tiles * 8 * a.dsplit), dim3(256), 0, st, a, a.kmat,
-                           (const uint4*)a.xp, a.class_dirs, a.dir_vec);
+                           (const uint4*)a.xp, a.class_dirs, a.dir_vec, a.kpre);
     return hipGetLastError();
 }
 

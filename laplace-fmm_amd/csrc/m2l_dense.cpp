@@ -103,7 +103,7 @@ static uint16_t half_bits(float x)
     return b;
 }
 
-void pack_m2l16(const OpTables& T, std::vector<uint16_t>& out, double& kinv)
+void pack_m2l16(const OpTables& T, std::vector<uint16_t>& out, double& kinv, std::vector<int>& kpre)
 {
     const int N = T.N, NR = N * N;
     const int RT = (NR + 31) / 32, KC = (NR + 15) / 16;
@@ -132,6 +132,25 @@ void pack_m2l16(const OpTables& T, std::vector<uint16_t>& out, double& kinv)
                         out[base + lane * 8 + s] = half_bits((float)hi);
                         out[base + 64 * 8 + lane * 8 + s] = half_bits(lo);
                     }
+    // block prefixes: a 32 x 16 block whose largest |entry| is below 2^-11 of
+    // kmax keeps hi*hi only (its lo part and the hi*lo product are below
+    // 2^-22 kmax max|X|); below 2^-22 kmax it contributes nothing measurable
+    kpre.assign((size_t)kNumM2LDirs * RT * 2, 0);
+    const double t3 = std::ldexp(kmax, -11), t1 = std::ldexp(kmax, -22);
+    for (int d = 0; d < kNumM2LDirs; ++d)
+        for (int rt = 0; rt < RT; ++rt) {
+            int k3 = 0, k1 = 0;
+            for (int kc = 0; kc < KC; ++kc) {
+                double bm = 0.0;
+                for (int i = rt * 32; i < std::min(NR, rt * 32 + 32); ++i)
+                    for (int k = kc * 16; k < std::min(NR, kc * 16 + 16); ++k)
+                        bm = std::max(bm, std::fabs(K[((size_t)d * NR + i) * NR + k]));
+                if (bm >= t3) k3 = kc + 1;
+                if (bm >= t1) k1 = kc + 1;
+            }
+            kpre[((size_t)d * RT + rt) * 2] = k3;
+            kpre[((size_t)d * RT + rt) * 2 + 1] = std::max(k1, k3);
+        }
 }
 
 }  // namespace lfmm

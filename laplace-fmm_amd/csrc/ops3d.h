@@ -78,6 +78,7 @@ struct LeafArgs {
 // Matrix-core M2L of one level (m2l16_kernels.h).
 struct M2L16Args {
     const uint4* kmat;         // split-fp16 operator fragments [dir][RT][KC][2][64]
+    const int2* kpre;          // [dir][RT] (k3, k1): K chunks needing all products / hi*hi
     const double2* outer;      // normalised multipoles of the level
     uint4* xp;                 // packed sources [size + 1][KC][2][2] (box `size` is zero)
     const uint32_t* xmax;      // largest |coefficient| of the level (float bits)

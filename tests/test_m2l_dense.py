@@ -77,7 +77,7 @@ def test_dense_m2l_reflection_symmetry(lfmm):
 def test_m2l16_packing_reconstructs_operators(lfmm, N):
     NR = N * N
     RT, KC = (NR + 31) // 32, (NR + 15) // 16
-    pk, kinv = lfmm.host_m2l16_pack(N)
+    pk, kinv, kpre = lfmm.host_m2l16_pack(N)
     assert pk.size == 316 * RT * KC * 2 * 64 * 8
     assert kinv > 0 and np.log2(kinv) == np.round(np.log2(kinv))  # a power of two
     h = pk.view(np.float16).astype(np.float64).reshape(316, RT, KC, 2, 64, 8)
@@ -98,3 +98,27 @@ def test_m2l16_packing_reconstructs_operators(lfmm, N):
         assert np.all(np.abs(h[d, :, :, 0]) <= 2.0 ** 14)
         # hi alone is the fp16 rounding of the scaled entry
         assert np.allclose(h[d, :, :, 0], (want / kinv).astype(np.float32).astype(np.float16), rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("N", [5, 11])
+def test_m2l16_block_prefixes_bound_dropped_terms(lfmm, N):
+    """Blocks at or beyond k1 are below 2^-22 of the largest entry, blocks at
+    or beyond k3 below 2^-11; k3 <= k1 <= KC, and the prefixes are tight."""
+    NR = N * N
+    RT, KC = (NR + 31) // 32, (NR + 15) // 16
+    _, _, kpre = lfmm.host_m2l16_pack(N)
+    Ks = [lfmm.host_m2l_dense(N, d) for d in range(316)]
+    kmax = max(np.abs(K).max() for K in Ks)
+    for d in range(0, 316, 7):
+        P = np.zeros((RT * 32, KC * 16))
+        P[:NR, :NR] = np.abs(Ks[d])
+        B = P.reshape(RT, 32, KC, 16).max(axis=(1, 3))
+        for rt in range(RT):
+            k3, k1 = kpre[d, rt]
+            assert 0 <= k3 <= k1 <= KC
+            assert np.all(B[rt, k1:] < kmax * 2.0 ** -22)
+            assert np.all(B[rt, k3:] < kmax * 2.0 ** -11)
+            if k1 > 0:
+                assert B[rt, k1 - 1] >= kmax * 2.0 ** -22 or k1 == k3
+            if k3 > 0:
+                assert B[rt, k3 - 1] >= kmax * 2.0 ** -11
