@@ -3,16 +3,24 @@
 
 Workload (the 4M-particle 3D configuration of the metric): N = 4,194,304
 particles, positions uniform in [0,1)^3 then charges uniform in [-1,1]
-(seeded numpy generator, positions drawn first), sources = targets, order
-p = 10 (11 terms), uniform Morton octree with leaves at level 6 (at most 64
-particles per leaf), FP64 throughout.  One step = one full solve through the
-C ABI with inputs and outputs resident in HBM: extent, keys, radix sort, tree
-levels, P2M, M2M, M2L+L2L, P2P+L2P and the scatter back to input order.
+(seeded numpy generator, positions drawn first), sources = targets, uniform
+Morton octree with leaves at level 6 (at most 64 particles per leaf).  One
+step = one full solve through the C ABI with inputs and outputs resident in
+HBM: extent, keys, radix sort, tree levels, P2M, M2M, M2L+L2L, P2P+L2P and the
+scatter back to input order.
 
-Printed: one JSON line with particles/s, the accuracy against an FP64 direct
-sum on sampled targets (self term excluded), the M2L roofline (kernel time
-from HIP events recorded by the library on its stream during the timed
-steps) and a CPU baseline (the CPU oracle on a bounded sample).
+The metric is quoted at rel-L2 potential error <= 1e-6 against the FP64
+direct sum, so the headline runs the smallest expansion order that meets it
+on this workload: 13 terms (p = 12), with the M2L of levels >= 4 on the
+matrix cores in split FP16 (everything else FP64).  The same line reports the
+p = 10 configuration (11 terms) and the all-FP64 path at both orders, each
+with its own measured error, all against one direct-sum reference on the same
+sampled targets.
+
+Printed: one JSON line with particles/s, the accuracy, the M2L roofline of
+the headline (kernel time from HIP events recorded by the library on its
+stream during the timed steps) and a CPU baseline (the CPU oracle on a
+bounded sample).
 """
 import argparse
 import json
@@ -28,7 +36,6 @@ sys.path.insert(0, ROOT)
 METRIC = "particles/sec (3D, 4M) at rel-L2 err ≤1e-6 vs FP64 direct; M2L MFMA util"
 FP64_PEAK_TFLOPS = 78.6
 FP16_PEAK_TFLOPS = 2500.0  # dense MFMA
-HBM_PEAK_GBS = 8000.0
 
 
 def m2l_flops_per_op(N):
@@ -40,11 +47,18 @@ def m2l_flops_per_op(N):
     return 2 * (6 * nx2 + rot) + 4 * sum(t * t for t in range(1, N + 1))
 
 
-def m2l16_flops_per_op(N):
-    """Algorithmic matrix-core flops of one split-FP16 M2L with N terms: the
-    dense N^2 x N^2 real operator times one expansion, three fp16 products
-    (hi*hi, hi*lo, lo*hi) per operand pair."""
-    return 3 * 2 * (N * N) ** 2
+def m2l16_mfma_flops_per_op(lfmm, N):
+    """Matrix-core flops issued per M2L op (one target column, one direction)
+    by k_m2l16: per row tile of 32, k1 + 2 k3 MFMAs of 32x32x16 (2 * 16384
+    flops for 32 columns), averaged over the directions as the 8 child classes
+    use them (each class its 189 offsets)."""
+    _, _, kpre = lfmm.host_m2l16_pack(N)  # (316, RT, 2) = (k3, k1)
+    KC = (N * N + 15) // 16
+    if N < 12:  # no block skipping below 12 terms: every block, three products
+        kpre = np.full_like(kpre, KC)
+    per_dir = (kpre[:, :, 1] + 2 * kpre[:, :, 0]).sum(axis=1) * 2 * 32 * 16  # flops per column
+    cd = lfmm.host_op_tables(N, "class_dirs").reshape(8, -1)
+    return float(per_dir[cd].mean())
 
 
 def main():
@@ -53,15 +67,18 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=4194304)
-    ap.add_argument("--terms", type=int, default=11)
+    ap.add_argument("--terms", type=int, default=13)
     ap.add_argument("--depth", type=int, default=7)
+    ap.add_argument("--fp64", action="store_true", help="headline on the all-FP64 path (default: mixed M2L)")
+    ap.add_argument("--mixed", action="store_true", help="(default) M2L of levels >= 4 on the matrix cores")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--sample", type=int, default=4096, help="targets checked against the direct sum")
-    ap.add_argument("--cpu-n", type=int, default=524288, help="particles in the CPU-baseline sample")
+    ap.add_argument("--cpu-n", type=int, default=0, help="particles in the CPU-baseline run (0: the full workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
-    ap.add_argument("--mixed", action="store_true", help="M2L of levels >= 4 on the matrix cores (split FP16)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the secondary configurations")
     args = ap.parse_args()
+    mixed = not args.fp64
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -82,119 +99,144 @@ def main():
     pot = torch.empty(args.n, dtype=torch.float64, device=dev)
     field = torch.empty((args.n, 3), dtype=torch.float64, device=dev)
     ctx = lfmm.Context(0)
+    idx = np.random.default_rng(args.seed + 1).choice(args.n, min(args.sample, args.n), replace=False)
+    idx_d = torch.from_numpy(idx).to(dev)
 
-    def step(timings=False):
-        return lfmm.fmm3d(xd, qd, terms=args.terms, tree_depth=args.depth, ctx=ctx, out=(pot, field),
-                          timings=timings, mixed=args.mixed)[2]
+    def run(terms, mix, steps, warmup):
+        def step(timings=False):
+            return lfmm.fmm3d(xd, qd, terms=terms, tree_depth=args.depth, ctx=ctx, out=(pot, field),
+                              timings=timings, mixed=mix)[2]
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    print("bench: warmup done (n=%d terms=%d depth=%d)" % (args.n, args.terms, args.depth), file=sys.stderr,
-          flush=True)
-    reports = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        reports.append(step(timings=True))
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms = 1e3 * elapsed / args.steps
-    value = args.n / (elapsed / args.steps)
-    print("bench: timed %d steps, %.3f ms/step" % (args.steps, ms), file=sys.stderr, flush=True)
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        reports = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            reports.append(step(timings=True))
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        sample = (pot[idx_d].cpu().numpy(), field[idx_d].cpu().numpy())
+        print("bench: terms=%d %s: %d steps, %.3f ms/step" % (terms, "mixed" if mix else "fp64", steps,
+                                                              1e3 * elapsed / steps), file=sys.stderr, flush=True)
+        return {"ms": 1e3 * elapsed / steps, "value": args.n / (elapsed / steps), "reports": reports,
+                "sample": sample}
 
+    head = run(args.terms, mixed, args.steps, args.warmup)
     # exact op counts (after the timed region)
     counts = lfmm.fmm3d(xd, qd, terms=args.terms, tree_depth=args.depth, ctx=ctx, out=(pot, field),
-                        timings=True, count_ops=True, mixed=args.mixed)[2]
+                        timings=True, count_ops=True, mixed=mixed)[2]
+    reports = head["reports"]
     m2l_ms = float(np.mean([r["t_m2l_kernels_ms"] for r in reports]))
-    launches = reports[0]["m2l_launches"]
     leaf_ms = float(np.mean([r["t_leaf_kernel_ms"] for r in reports]))
-    flops = counts["m2l_ops"] * m2l_flops_per_op(args.terms)
-    achieved = flops / (m2l_ms * 1e-3) / 1e12
-    traffic = None
+    if mixed:
+        # dominant kernel: the matrix-core M2L of levels >= 4
+        mx_ms = float(np.mean([r["t_m2l_mixed_ms"] for r in reports]))
+        mx_launches = reports[0]["m2l_mixed_launches"]
+        mx_ops = counts["m2l_ops_mixed"]
+        fpo = m2l16_mfma_flops_per_op(lfmm, args.terms)
+        ach = mx_ops * fpo / (mx_ms * 1e-3) / 1e12
+        roofline = {"kernel": "k_m2l16<N> (split-FP16 MFMA M2L, levels >= 4, one launch per level)", "bound": "mfma",
+                    "achieved": round(ach, 3), "peak": FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(ach / FP16_PEAK_TFLOPS, 4), "traffic": None,
+                    "m2l_ops": mx_ops, "flops_per_op": round(fpo, 1), "flops_counted": "MFMA tile flops issued",
+                    "launches": mx_launches, "avg_launch_ms": round(mx_ms / max(mx_launches, 1), 4),
+                    "m2l_kernels_ms_per_step": round(mx_ms, 4),
+                    "precision": "fp16 hi/lo operands (22 bits), fp32 accumulation",
+                    "fp64_levels": {"m2l_ops": counts["m2l_ops"] - mx_ops, "ms_per_step": round(m2l_ms - mx_ms, 4)},
+                    "fp64_equivalent_tflops": round(mx_ops * m2l_flops_per_op(args.terms) / (mx_ms * 1e-3) / 1e12, 3)}
+    else:
+        launches = reports[0]["m2l_launches"]
+        ach = counts["m2l_ops"] * m2l_flops_per_op(args.terms) / (m2l_ms * 1e-3) / 1e12
+        roofline = {"kernel": "k_rtr<N,M2L> (M2L+L2L, one launch per level)", "bound": "mfma",
+                    "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                    "m2l_ops": counts["m2l_ops"], "flops_per_op": m2l_flops_per_op(args.terms),
+                    "launches": launches, "avg_launch_ms": round(m2l_ms / max(launches, 1), 4),
+                    "m2l_kernels_ms_per_step": round(m2l_ms, 4), "precision": "fp64 VALU"}
     tf = os.path.join(ROOT, "profiles", "m2l_traffic.json")
     if os.path.exists(tf):
         try:
             t = json.load(open(tf))
             if (t.get("n") == args.n and t.get("terms") == args.terms and t.get("tree_depth") == args.depth
-                    and bool(t.get("mixed", False)) == args.mixed):
-                traffic = t.get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
-    if args.mixed:
-        # dominant kernel: the matrix-core M2L of levels >= 4
-        mx_ms = float(np.mean([r["t_m2l_mixed_ms"] for r in reports]))
-        mx_launches = reports[0]["m2l_mixed_launches"]
-        mx_ops = counts["m2l_ops_mixed"]
-        mx_ach = mx_ops * m2l16_flops_per_op(args.terms) / (mx_ms * 1e-3) / 1e12
-        roofline = {"kernel": "k_m2l16<N> (split-FP16 MFMA M2L, levels >= 4, one launch per level)", "bound": "mfma",
-                    "achieved": round(mx_ach, 3), "peak": FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(mx_ach / FP16_PEAK_TFLOPS, 4), "traffic": traffic,
-                    "m2l_ops": mx_ops, "flops_per_op": m2l16_flops_per_op(args.terms), "launches": mx_launches,
-                    "avg_launch_ms": round(mx_ms / max(mx_launches, 1), 4), "m2l_kernels_ms_per_step": round(mx_ms, 4),
-                    "precision": "fp16 hi/lo operands (22 bits), fp32 accumulation",
-                    "fp64_levels": {"m2l_ops": counts["m2l_ops"] - mx_ops, "ms_per_step": round(m2l_ms - mx_ms, 4)},
-                    "fp64_equivalent_tflops": round(mx_ops * m2l_flops_per_op(args.terms) / (mx_ms * 1e-3) / 1e12, 3)}
-    else:
-        roofline = {"kernel": "k_rtr<N,M2L> (M2L+L2L, one launch per level)", "bound": "mfma",
-                    "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
-                    "m2l_ops": counts["m2l_ops"], "flops_per_op": m2l_flops_per_op(args.terms),
-                    "launches": launches, "avg_launch_ms": round(m2l_ms / max(launches, 1), 4),
-                    "m2l_kernels_ms_per_step": round(m2l_ms, 4), "precision": "fp64 VALU"}
-    # near-field kernel as HBM-bound figure: 32 B per source read + 32 B per target in/out, per neighbour list
+                    and bool(t.get("mixed", False)) == mixed):
+                roofline["traffic"] = t.get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            pass
     p2p_pairs = counts["p2p_pairs"]
     leaf = {"kernel": "k_leaf<N> (P2P+L2P)", "ms_per_step": round(leaf_ms, 4), "p2p_pairs": p2p_pairs,
             "gflops_p2p": round(20.0 * p2p_pairs / (leaf_ms * 1e-3) / 1e9, 1)}
 
+    # secondary configurations: p = 10 (11 terms) and the all-FP64 path
+    extras = []
+    if not args.no_extra:
+        for terms, mix in ((11, True), (args.terms, False), (11, False)):
+            if (terms, mix) == (args.terms, mixed):
+                continue
+            r = run(terms, mix, 3, 1)
+            extras.append({"terms": terms, "m2l": "split-FP16 MFMA (levels >= 4)" if mix else "FP64", "r": r})
+
     accuracy = None
+    other = []
     if not args.no_error:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
-        torch.cuda.synchronize()
-        idx = np.random.default_rng(args.seed + 1).choice(args.n, min(args.sample, args.n), replace=False)
-        p = pot.cpu().numpy()[idx]
-        f = field.cpu().numpy()[idx]
         t1 = time.perf_counter()
         dp, df = oracle.direct3d(x, q, tgt_index=idx, include_self=False)
         print("bench: direct sum on %d targets took %.1f s" % (len(idx), time.perf_counter() - t1), file=sys.stderr,
               flush=True)
-        phys = p - q[idx] * 1e4  # remove the reference's clamped self term q_i / 1e-4
-        ep = float(np.linalg.norm(phys - dp) / np.linalg.norm(dp))
-        ef = float(np.linalg.norm(f - df) / np.linalg.norm(df))
-        dpi = dp + q[idx] * 1e4
+
+        def errors(sample):
+            p, f = sample
+            phys = p - q[idx] * 1e4  # remove the reference's clamped self term q_i / 1e-4
+            return (float(np.linalg.norm(phys - dp) / np.linalg.norm(dp)),
+                    float(np.linalg.norm(f - df) / np.linalg.norm(df)))
+
+        ep, ef = errors(head["sample"])
         accuracy = {"targets": int(len(idx)), "rel_l2_potential": ep, "rel_l2_field": ef,
                     "meets_1e-6": bool(ep <= 1e-6),
                     "reference_convention": dict(zip(("max_err", "l2_err", "rel_err"),
-                                                     map(float, oracle.read_error(dpi, p))))}
+                                                     map(float, oracle.read_error(dp + q[idx] * 1e4,
+                                                                                  head["sample"][0]))))}
+        for e in extras:
+            e["rel_l2_potential"], e["rel_l2_field"] = errors(e["r"]["sample"])
+    for e in extras:
+        r = e.pop("r")
+        e.update({"ms_per_step": round(r["ms"], 3), "value": r["value"],
+                  "meets_1e-6": bool(e.get("rel_l2_potential", 1.0) <= 1e-6)})
+        other.append(e)
 
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
-        rs = np.random.default_rng(args.seed + 7)
-        xs = rs.random((args.cpu_n, 3))
-        qs = rs.uniform(-1.0, 1.0, args.cpu_n)
-        cdepth = max(2, args.depth - int(round(np.log(args.n / args.cpu_n) / np.log(8.0))))
+        cn = args.cpu_n if args.cpu_n > 0 else args.n
+        if cn == args.n:  # the same particles
+            xs, qs = x, q
+        else:
+            rs = np.random.default_rng(args.seed + 7)
+            xs = rs.random((cn, 3))
+            qs = rs.uniform(-1.0, 1.0, cn)
+        cdepth = max(2, args.depth - int(round(np.log(args.n / cn) / np.log(8.0))))
         t1 = time.perf_counter()
         oracle.fmm3d(xs, qs, args.terms, cdepth)
         tc = time.perf_counter() - t1
-        cpu = {"value": args.cpu_n / tc, "unit": "particles/s", "cores": oracle.num_threads(), "kind": "port",
-               "sample": "CPU oracle (C/OpenMP restatement of the reference CPU FMM) on %d uniform particles, "
-                         "%d terms, depth %d (same leaf density), one solve of %.1f s" % (args.cpu_n, args.terms,
-                                                                                         cdepth, tc)}
+        cpu = {"value": cn / tc, "unit": "particles/s", "cores": oracle.num_threads(), "kind": "port",
+               "sample": "CPU oracle (C/OpenMP restatement of the reference CPU FMM, FP64) on %d uniform particles, "
+                         "%d terms, depth %d, one solve of %.1f s" % (cn, args.terms, cdepth, tc)}
         print("bench: cpu baseline %.1f s" % tc, file=sys.stderr, flush=True)
 
-    out = {"metric": METRIC, "value": value, "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f64+f16x2" if args.mixed else "f64", "data": "synthetic",
+    out = {"metric": METRIC, "value": head["value"], "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": head["ms"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64+f16x2" if mixed else "f64", "data": "synthetic",
            "config": {"workload": "3D Laplace FMM, %d particles uniform in [0,1)^3, q~U[-1,1], p=%d (%d terms), "
                                   "uniform octree depth %d (leaves at level %d)" % (args.n, args.terms - 1, args.terms,
                                                                                      args.depth, args.depth - 1),
                       "n": args.n, "terms": args.terms, "tree_depth": args.depth, "max_leaf": counts["max_leaf"],
-                      "m2l": ("split-FP16 MFMA at levels >= 4, FP64 at levels 2-3" if args.mixed else "FP64"),
+                      "m2l": ("split-FP16 MFMA at levels >= 4, FP64 at levels 2-3" if mixed else "FP64"),
                       "parallelism": "single GPU"},
-           "accuracy": accuracy, "roofline": roofline, "leaf_kernel": leaf, "cpu_baseline": cpu}
+           "accuracy": accuracy, "roofline": roofline, "leaf_kernel": leaf, "other_configs": other,
+           "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 
