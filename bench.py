@@ -49,14 +49,16 @@ def m2l_flops_per_op(N):
 
 def m2l16_mfma_flops_per_op(lfmm, N):
     """Matrix-core flops issued per M2L op (one target column, one direction)
-    by k_m2l16: per row tile of 32, k1 + 2 k3 MFMAs of 32x32x16 (2 * 16384
-    flops for 32 columns), averaged over the directions as the 8 child classes
-    use them (each class its 189 offsets)."""
+    by k_m2l16r: per K chunk of 16 and row tile of 32, one hi*hi MFMA of
+    32x32x16 (2 * 16384 flops for 32 columns) for every chunk, and hi*lo +
+    lo*hi for the chunks below the direction's bound k3 (the largest k3 of
+    its row tiles); averaged over the directions as the 8 child classes use
+    them (each class its 189 offsets)."""
     _, _, kpre = lfmm.host_m2l16_pack(N)  # (316, RT, 2) = (k3, k1)
+    RT = kpre.shape[1]
     KC = (N * N + 15) // 16
-    if N < 12:  # no block skipping below 12 terms: every block, three products
-        kpre = np.full_like(kpre, KC)
-    per_dir = (kpre[:, :, 1] + 2 * kpre[:, :, 0]).sum(axis=1) * 2 * 32 * 16  # flops per column
+    k3 = kpre[:, :, 0].max(axis=1)
+    per_dir = (RT * KC + 2 * RT * k3) * 2 * 32 * 16  # flops per column
     cd = lfmm.host_op_tables(N, "class_dirs").reshape(8, -1)
     return float(per_dir[cd].mean())
 
@@ -136,7 +138,7 @@ def main():
         mx_ops = counts["m2l_ops_mixed"]
         fpo = m2l16_mfma_flops_per_op(lfmm, args.terms)
         ach = mx_ops * fpo / (mx_ms * 1e-3) / 1e12
-        roofline = {"kernel": "k_m2l16<N> (split-FP16 MFMA M2L, levels >= 4, one launch per level)", "bound": "mfma",
+        roofline = {"kernel": "k_m2l16r<N> (split-FP16 MFMA M2L, levels >= 4, one launch per level)", "bound": "mfma",
                     "achieved": round(ach, 3), "peak": FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(ach / FP16_PEAK_TFLOPS, 4), "traffic": None,
                     "m2l_ops": mx_ops, "flops_per_op": round(fpo, 1), "flops_counted": "MFMA tile flops issued",

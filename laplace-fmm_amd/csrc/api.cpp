@@ -427,11 +427,15 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                 m.kinv = DT->m16_kinv;
                 m.dst = inner[l];
                 m.tiles = (int)((cnt[l - 1] + m16_targets(N) - 1) / m16_targets(N));
-                m.dsplit = dsplit_for((int64_t)m.tiles * 8);
+                // the row-complete kernel holds one workgroup per CU: 512
+                // workgroups fill the chip twice (fewer parts to sum)
+                m.dsplit = m16_rowfull(N) ? (int)std::max<int64_t>(1, std::min<int64_t>(32, (512 + m.tiles * 8 - 1) /
+                                                                                                 (m.tiles * 8)))
+                                          : dsplit_for((int64_t)m.tiles * 8);
                 if (m.dsplit > 1) {
+                    // every part writes its whole slice (no clearing needed)
                     m.partial = take<double2>(A, (int64_t)m.dsplit * cnt[l] * NX);
                     if (!m.partial) return LFMM_ERR_NOMEM;
-                    LFMM_HIP(fill_u32((uint32_t*)m.partial, (int64_t)m.dsplit * cnt[l] * NX * 4, 0u, st));
                 }
                 LFMM_HIP(launch_m2l16_pack(N, m, st));
                 LFMM_HIP(launch_rtr(N, 2, DT->rt, L, st));

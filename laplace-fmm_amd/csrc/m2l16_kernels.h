@@ -32,6 +32,7 @@ namespace lfmm {
 
 typedef _Float16 m16_h8 __attribute__((ext_vector_type(8)));
 typedef float m16_acc __attribute__((ext_vector_type(16)));
+typedef unsigned int m16_u4 __attribute__((ext_vector_type(4)));
 
 template <int N>
 struct M16 {
@@ -313,6 +314,12 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
         const int t = target_of(ct * 32 + (lane & 31), tx, ty, tz);
         if (t < 0) continue;
         double* out = (double*)(dst + (int64_t)t * tri(N));
+        if (a.dsplit > 1 && w == 0) {
+            // the part's slice is written whole: the imaginary parts of the
+            // m = 0 coefficients (no real coordinate) as zeros; lanes l and
+            // l + 32 hold the same target
+            for (int n = lane >> 5; n < N; n += 2) out[2 * tri(n) + 1] = 0.0;
+        }
 #pragma unroll
         for (int r = 0; r < RW; ++r) {
             const int rt = w + 4 * r;
@@ -323,6 +330,294 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
                 if (row < NR) {
                     const int o = m16_coef_offset(row);
                     const double val = (double)acc[r][ct][v] * inv;
+                    if (a.dsplit > 1) out[o] = val;
+                    else out[o] += val;
+                }
+            }
+        }
+    }
+}
+
+// ===================================================================== row-complete variant
+// k_m2l16r: every wave computes all row tiles of its own CTW column tiles, so
+// the four waves of a workgroup need the same operator fragments and share
+// them through LDS; each wave loads its own source fragments straight into
+// registers.  Per direction, the operator is streamed in NSEG segments of SK
+// K chunks (segment s holds chunks s, s + NSEG, s + 2 NSEG, ...: heavy low
+// chunks spread over all segments).  Segment i + 1 is loaded (global ->
+// registers) while segment i is multiplied and written to the other LDS
+// buffer afterwards: one barrier per segment, no wave waits for another's
+// rows, and each operator fragment read from LDS feeds 2 * CTW or 4 * CTW
+// MFMAs.  Skipping is per direction instead of per block, which keeps the
+// MFMA stream free of branches: hi*hi for every chunk below the largest k1
+// of the direction's row tiles, hi*lo + lo*hi below the largest k3.  That
+// multiplies a superset of the blocks k_m2l16 multiplies, so the error bound
+// of k_m2l16 holds.
+template <int N>
+struct M16R {
+    static constexpr int RT = M16<N>::RT, KC = M16<N>::KC;
+    static constexpr int SK = 3;                          // K chunks per segment
+    static constexpr int NSEG = (KC + SK - 1) / SK;       // segments per direction
+    static constexpr int SF = SK * RT * 2;                // fragments per segment (hi, lo)
+    static constexpr int SFW = (SF + 3) / 4;              // fragments staged per wave
+};
+
+// Column tiles per wave: two while the accumulators (RT * CTW * 16 floats
+// per lane) leave room for the operand pipeline, else one.
+constexpr int m16r_ctw(int N) { return m16r_max_rt2 >= (N * N + 31) / 32 ? 2 : 1; }
+
+template <int N, int CTW>
+__global__ void __launch_bounds__(256) k_m2l16r(M2L16Args a, const uint4* __restrict__ kmat, const uint4* __restrict__ xp,
+                                                const int* __restrict__ class_dirs, const int* __restrict__ dir_vec,
+                                                const int2* __restrict__ kpre)
+{
+    using S = M16<N>;
+    using R = M16R<N>;
+    constexpr int RT = S::RT, KC = S::KC, NR = S::NR, PIECES = S::PIECES;
+    constexpr int SK = R::SK, NSEG = R::NSEG, SF = R::SF, SFW = R::SFW;
+    constexpr int TC = 4 * 32 * CTW;  // targets (columns) per workgroup
+    // fragments move as native 16-byte vectors (plain loads and stores, no
+    // aggregate copies, so the staging arrays stay in registers)
+    __shared__ m16_u4 as[2][SF][64];
+    const m16_u4* __restrict__ kv = (const m16_u4*)kmat;
+    const m16_u4* __restrict__ xv = (const m16_u4*)xp;
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int blk = xcd_local_block(blockIdx.x, gridDim.x);
+    const int tile = blk / (8 * a.dsplit);
+    const int part = (blk - tile * 8 * a.dsplit) >> 3;
+    const int c = blk & 7;
+    const int d_beg = part * kDirsPerClass / a.dsplit, d_end = (part + 1) * kDirsPerClass / a.dsplit;
+    const int side = 1 << a.level;
+
+    // targets of this lane: the class-c child of parent tile * TC + column,
+    // column 32 * (CTW * w + ct) + lane % 32
+    int tgt[CTW];
+    uint32_t tx[CTW], ty[CTW], tz[CTW];
+#pragma unroll
+    for (int ct = 0; ct < CTW; ++ct) {
+        const int P = tile * TC + 32 * (CTW * w + ct) + (lane & 31);
+        int t = -1;
+        if (P < a.num_parents) {
+            const int cb = a.child_begin[P], ce = a.child_begin[P + 1];
+            for (int i = cb; i < ce; ++i)
+                if ((int)(a.child_keys[i] & 7u) == c) t = i;
+        }
+        tgt[ct] = t;
+        tx[ct] = ty[ct] = tz[ct] = 0;
+        if (t >= 0) {
+            const uint64_t k = a.child_keys[t];
+            tx[ct] = compact3(k >> 2);
+            ty[ct] = compact3(k >> 1);
+            tz[ct] = compact3(k);
+        }
+    }
+    auto dir_of = [&](int d) { return class_dirs[c * kDirsPerClass + d]; };
+    // source box of column tile ct in direction dir (the zero box when absent)
+    auto box_of = [&](int dir, int ct) -> int {
+        int box = a.size;
+        if (tgt[ct] >= 0) {
+            const int sx = (int)tx[ct] + dir_vec[4 * dir], sy = (int)ty[ct] + dir_vec[4 * dir + 1],
+                      sz = (int)tz[ct] + dir_vec[4 * dir + 2];
+            if (sx >= 0 && sy >= 0 && sz >= 0 && sx < side && sy < side && sz < side) {
+                const int s = find_box(morton3(sx, sy, sz), a.lut, a.keys, a.size);
+                if (s >= 0) box = s;
+            }
+        }
+        return box;
+    };
+    // bounds of direction dir (wave-uniform): K chunks below k1 need hi*hi,
+    // below k3 all three products, for some row tile
+    auto prefixes = [&](int dir, int& bk3, int& bk1) {
+        bk3 = 0;
+        bk1 = 0;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+            const int2 v = kpre[dir * RT + rt];
+            bk3 = max(bk3, __builtin_amdgcn_readfirstlane(v.x));
+            bk1 = max(bk1, __builtin_amdgcn_readfirstlane(v.y));
+        }
+    };
+    // operator fragments of segment s of direction dir: wave w loads slot
+    // fragments w, w + 4, ...  (slot fragment f = (kci * RT + rt) * 2 + hl);
+    // blocks the precision does not need come from one cache-resident block
+    struct Stage {
+        m16_u4 v[SFW];
+    };
+    auto stage_a = [&](int dir, int s, int k3d, int k1d) -> Stage {
+        Stage stg;
+#pragma unroll
+        for (int j = 0; j < SFW; ++j) {
+            const int f = w + 4 * j;
+            if ((SF & 3) == 0 || f < SF) {
+                const int kci = f / (2 * RT), rt = (f >> 1) % RT, hl = f & 1;
+                const int kc = s + NSEG * kci;
+                const bool need = kc < KC && kc < (hl ? k3d : k1d);
+                const int64_t b = need ? ((int64_t)(dir * RT + rt) * KC + kc) * 2 + hl : 0;
+                stg.v[j] = kv[b * 64 + lane];
+            }
+        }
+        return stg;
+    };
+    auto store_a = [&](m16_u4* slot, const Stage& stg) {
+#pragma unroll
+        for (int j = 0; j < SFW; ++j) {
+            const int f = w + 4 * j;
+            if ((SF & 3) == 0 || f < SF) slot[f * 64 + lane] = stg.v[j];
+        }
+    };
+    // source fragments of chunk kc (both column tiles): hi below the
+    // direction's bound k1 (else the zero box), lo only below its bound k3
+    // (the lo products are not issued beyond it)
+    auto load_b = [&](const int* box, int kc, int bk3, int bk1, m16_u4 (&b)[CTW][2]) {
+#pragma unroll
+        for (int ct = 0; ct < CTW; ++ct) {
+            const int bh = kc < bk1 ? box[ct] : a.size;
+            b[ct][0] = xv[(int64_t)bh * PIECES + kc * 4 + (lane >> 5)];
+        }
+        if (kc < bk3) {
+#pragma unroll
+            for (int ct = 0; ct < CTW; ++ct) b[ct][1] = xv[(int64_t)box[ct] * PIECES + kc * 4 + 2 + (lane >> 5)];
+        }
+    };
+    // steps e = s * SK + kci of a direction in execution order (chunk
+    // kc = s + NSEG * kci, absent when kc >= KC); NSTEP is a multiple of 3,
+    // so a 3-slot ring of source fragments keeps its slots across directions
+    constexpr int NSTEP = NSEG * SK;
+    static_assert(NSTEP % 3 == 0, "source ring");
+    auto kc_of = [](int e) { return e / SK + NSEG * (e % SK); };
+
+    // accumulators start as the product of zero operands, so they are
+    // born in (and stay in) accumulation registers across the direction loop
+    m16_acc acc[RT][CTW];
+    {
+        const m16_h8 z8 = {};
+        const m16_acc z16 = {};
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int ct = 0; ct < CTW; ++ct) acc[r][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(z8, z8, z16, 0, 0, 0);
+    }
+
+    int dcur = dir_of(d_beg);
+    int bk3, bk1;
+    prefixes(dcur, bk3, bk1);
+    int bx[CTW];
+#pragma unroll
+    for (int ct = 0; ct < CTW; ++ct) bx[ct] = box_of(dcur, ct);
+    Stage stg;
+    m16_u4 br[3][CTW][2];  // source fragments of steps e, e + 1, e + 2
+    m16_u4* cur = &as[0][0][0];
+    m16_u4* nxt = &as[1][0][0];
+    stg = stage_a(dcur, 0, bk3, bk1);
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+        if (kc_of(e) < KC) load_b(bx, kc_of(e), bk3, bk1, br[e]);
+    store_a(cur, stg);
+    lds_barrier();
+    for (int d = d_beg; d < d_end; ++d) {
+        const int dnext = dir_of(min(d + 1, d_end - 1));  // the last step re-stages its own data (never read)
+        int bxn[CTW];
+#pragma unroll
+        for (int ct = 0; ct < CTW; ++ct) bxn[ct] = box_of(dnext, ct);
+        int bk3n, bk1n;
+        prefixes(dnext, bk3n, bk1n);
+#pragma unroll
+        for (int s = 0; s < NSEG; ++s) {
+            // the next segment's operator fragments, in flight under this
+            // segment's MFMAs (written to the other LDS buffer at its end)
+            if (s + 1 < NSEG) stg = stage_a(dcur, s + 1, bk3, bk1);
+            else stg = stage_a(dnext, 0, bk3n, bk1n);
+            // Per chunk: hi*hi for every row tile (beyond the bound k1 the
+            // sources are the zero box, so those products add nothing), the
+            // two lo products below the bound k3.  The next chunk's hi
+            // fragments and this chunk's lo fragments are read from LDS while
+            // the hi*hi products run; sources arrive two steps ahead.
+            m16_u4 fh[2][RT];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) fh[0][rt] = cur[(rt * 2) * 64 + lane];
+#pragma unroll
+            for (int kci = 0; kci < SK; ++kci) {
+                const int e = s * SK + kci, kc = kc_of(e);
+                {
+                    const int e2 = e + 2;
+                    if (e2 < NSTEP) {
+                        if (kc_of(e2) < KC) load_b(bx, kc_of(e2), bk3, bk1, br[e2 % 3]);
+                    } else if (kc_of(e2 - NSTEP) < KC) {
+                        load_b(bxn, kc_of(e2 - NSTEP), bk3n, bk1n, br[e2 % 3]);
+                    }
+                }
+                if (kc < KC) {
+                    const m16_u4(&b)[CTW][2] = br[e % 3];
+                    const int h = kci & 1;
+                    m16_u4 fl[RT];
+#pragma unroll
+                    for (int rt = 0; rt < RT; ++rt) fl[rt] = cur[((kci * RT + rt) * 2 + 1) * 64 + lane];
+                    if (kci + 1 < SK && kc + NSEG < KC) {
+#pragma unroll
+                        for (int rt = 0; rt < RT; ++rt) fh[h ^ 1][rt] = cur[(((kci + 1) * RT + rt) * 2) * 64 + lane];
+                    }
+#pragma unroll
+                    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                        for (int ct = 0; ct < CTW; ++ct)
+                            acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                                __builtin_bit_cast(m16_h8, fh[h][rt]), __builtin_bit_cast(m16_h8, b[ct][0]), acc[rt][ct],
+                                0, 0, 0);
+                    if (kc < bk3) {
+#pragma unroll
+                        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                            for (int ct = 0; ct < CTW; ++ct) {
+                                acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                                    __builtin_bit_cast(m16_h8, fh[h][rt]), __builtin_bit_cast(m16_h8, b[ct][1]),
+                                    acc[rt][ct], 0, 0, 0);
+                                acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                                    __builtin_bit_cast(m16_h8, fl[rt]), __builtin_bit_cast(m16_h8, b[ct][0]),
+                                    acc[rt][ct], 0, 0, 0);
+                            }
+                    }
+                }
+                // keep each chunk's reads and products together (bounded
+                // register lifetimes)
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            store_a(nxt, stg);
+            m16_u4* t = cur;
+            cur = nxt;
+            nxt = t;
+            lds_barrier();
+        }
+        dcur = dnext;
+        bk3 = bk3n;
+        bk1 = bk1n;
+#pragma unroll
+        for (int ct = 0; ct < CTW; ++ct) bx[ct] = bxn[ct];
+    }
+
+    // ---- epilogue: L += acc / (scale of K * scale of X) in FP64, or the
+    // part's partial sum when the directions are split (every double of the
+    // part's slice is written: the imaginary parts of the m = 0 coefficients,
+    // which have no real coordinate, as zeros)
+    const double inv = a.kinv / m16_xscale(*a.xmax);
+    double2* const dst = a.dsplit > 1 ? a.partial + (int64_t)part * a.size * tri(N) : a.dst;
+#pragma unroll
+    for (int ct = 0; ct < CTW; ++ct) {
+        if (tgt[ct] < 0) continue;
+        double* out = (double*)(dst + (int64_t)tgt[ct] * tri(N));
+        if (a.dsplit > 1) {
+            // lanes l and l + 32 hold the same target
+            for (int n = lane >> 5; n < N; n += 2) out[2 * tri(n) + 1] = 0.0;
+        }
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = rt * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                if (row < NR) {
+                    const int o = m16_coef_offset(row);
+                    const double val = (double)acc[rt][ct][v] * inv;
                     if (a.dsplit > 1) out[o] = val;
                     else out[o] += val;
                 }
@@ -347,12 +642,17 @@ hipError_t do_m2l16_pack(const M2L16Args& a, hipStream_t st)
 template <int N>
 hipError_t do_m2l16(const M2L16Args& a, hipStream_t st)
 {
-    static_assert(m16_targets(N) == 32 * m16_ct(N), "tile width");
+    static_assert(m16_targets(N) == (m16_rowfull(N) ? 128 * m16r_ctw(N) : 32 * m16_ct(N)), "tile width");
     const int tiles = (a.num_parents + m16_targets(N) - 1) / m16_targets(N);
     if (tiles != a.tiles || a.dsplit < 1 || a.dsplit > kDirsPerClass) return hipErrorInvalidValue;
-    if (tiles > 0)
-        hipLaunchKernelGGL((k_m2l16<N, m16_ct(N)>), dim3(tiles * 8 * a.dsplit), dim3(256), 0, st, a, a.kmat,
-                           (const uint4*)a.xp, a.class_dirs, a.dir_vec, a.kpre);
+    if (tiles > 0) {
+        if constexpr (m16_rowfull(N))
+            hipLaunchKernelGGL((k_m2l16r<N, m16r_ctw(N)>), dim3(tiles * 8 * a.dsplit), dim3(256), 0, st, a, a.kmat,
+                               (const uint4*)a.xp, a.class_dirs, a.dir_vec, a.kpre);
+        else
+            hipLaunchKernelGGL((k_m2l16<N, m16_ct(N)>), dim3(tiles * 8 * a.dsplit), dim3(256), 0, st, a, a.kmat,
+                               (const uint4*)a.xp, a.class_dirs, a.dir_vec, a.kpre);
+    }
     return hipGetLastError();
 }
 

@@ -98,9 +98,20 @@ struct M2L16Args {
     double2* partial = nullptr;
 };
 
-// Targets per workgroup of the matrix-core M2L kernel (two column tiles of
-// 32 while the double-buffered source stage fits the LDS, one above N = 16).
-constexpr int m16_targets(int N) { return N <= 16 ? 64 : 32; }
+// Matrix-core M2L kernel per expansion size: the row-complete k_m2l16r
+// (operator shared through LDS, 128 or 256 targets per workgroup) from
+// LFMM_M16R_MIN_N terms on, below it k_m2l16 (sources shared through LDS, 64
+// or 32 targets per workgroup).
+#ifndef LFMM_M16R_MIN_N
+#define LFMM_M16R_MIN_N 1
+#endif
+constexpr bool m16_rowfull(int N) { return N >= LFMM_M16R_MIN_N; }
+// k_m2l16r keeps two column tiles per wave up to this many row tiles of 32
+constexpr int m16r_max_rt2 = 7;
+constexpr int m16_targets(int N)
+{
+    return m16_rowfull(N) ? ((N * N + 31) / 32 <= m16r_max_rt2 ? 256 : 128) : (N <= 16 ? 64 : 32);
+}
 
 hipError_t launch_p2m(int N, const P2MArgs& a, hipStream_t st);
 hipError_t launch_rtr(int N, int kind, const RtrTables& T, const RtrLevel& L, hipStream_t st);

@@ -29,7 +29,7 @@ def blobs(n, seed):
     return x, rng.uniform(-1, 1, n)
 
 
-@pytest.mark.parametrize("N", [4, 9, 11, 12, 14, 17, 18])
+@pytest.mark.parametrize("N", [1, 2, 4, 9, 11, 12, 13, 14, 15, 17, 18])
 def test_mixed_matches_fp64(lfmm, request, N):
     case_name(request)
     x, q = uniform_cube(40000, 300 + N)
