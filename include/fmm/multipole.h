@@ -72,3 +72,6 @@ private:
 
 }  // namespace gpu
 }  // namespace fmm
+
+// the checking path (direct sums, exact-result files, ReadError)
+#include "exact.h"

@@ -180,6 +180,50 @@ int lfmm_tree2d(lfmm_ctx* ctx, const lfmm_points2d* particles, int tree_depth, i
                 int32_t* depth_out, int64_t* level_sizes, double* geom, int64_t* pos_map,
                 uint64_t* keys, int64_t* begins);
 
+/*
+ * FP64 direct sums over all sources for every target: replaces
+ * fmm::gpu::ComputeExact<InteractionType> (src/common/simple_math.cu:24-130;
+ * CPU twin fmm::ComputeExact, src/common/simple_math.h:218-345), which the
+ * reference's drivers call to check an FMM run (src/examples/main.cpp:129-130,
+ * 183-184).  Pair terms as the FMM's (src/common/simple_math.h:120-151):
+ *   3D  potential q / max(|r|, 1e-4), field q r / max(|r|, 1e-4)^3;
+ *   2D  potential q * 0.5 log max(|dz|^2, 1e-8), force q conj(dz) / max(|dz|^2, 1e-8),
+ * r = x_i - x_j, summed over j in input order.  The self pair is included (as
+ * in the reference) unless LFMM_FLAG_EXCLUDE_SELF is given in a single-set
+ * call (targets NULL); other flags are ignored.  Outputs as lfmm_fmm3d /
+ * lfmm_fmm2d (field n_out*3 in 3D, n_out*2 = complex force in 2D).  Cost is
+ * n_out * n_sources pair terms: a checking tool, not a solver.
+ */
+int lfmm_direct3d(lfmm_ctx* ctx, const lfmm_points3d* sources, const lfmm_points3d* targets, int memory,
+                  unsigned flags, double* potential, double* field);
+int lfmm_direct2d(lfmm_ctx* ctx, const lfmm_points2d* sources, const lfmm_points2d* targets, int memory,
+                  unsigned flags, double* potential, double* field);
+
+/* Result kinds, in the order of the reference's fmm::InteractionType
+ * (src/common/utils.h:19-24). */
+typedef enum lfmm_interaction {
+    LFMM_POTENTIAL2D = 0,  /* one double per value                  */
+    LFMM_FORCE2D = 1,      /* complex: (re, im)                     */
+    LFMM_POTENTIAL3D = 2,  /* one double per value                  */
+    LFMM_FORCE3D = 3       /* vector: (x, y, z)                     */
+} lfmm_interaction;
+
+/* The reference's exact-result files ("potential3d_exact.txt", ...; names
+ * from src/common/simple_math.cu:103-119).  lfmm_write_exact writes n values
+ * one per line with 20 significant digits, a complex as "(re,im)" and a
+ * vector as "x y z" (src/common/simple_math.cu:121-129, utils.h:108-109);
+ * lfmm_read_exact reads them back (src/common/simple_math.h:374-376) and
+ * returns the count read or a negative status.  Host memory; no context. */
+const char* lfmm_exact_filename(int kind);
+int lfmm_write_exact(const char* path, int kind, int64_t n, const double* values);
+int64_t lfmm_read_exact(const char* path, int kind, int64_t n, double* values);
+/* The figures of the reference's ReadError (src/common/simple_math.h:378-401):
+ * max |e|, the "l2 error" sqrt(mean((|e_i| / |exact_i|)^2)) and the
+ * "relative error" sum |e_i| / sum |exact_i|, |.| the absolute value, complex
+ * modulus or Euclidean norm.  Any output pointer may be NULL. */
+int lfmm_error_stats(int kind, int64_t n, const double* exact, const double* result, double* max_err,
+                     double* l2_err, double* rel_err);
+
 /* Inspection hooks of the host FP64 table builder (no reference
  * counterpart; used by the CPU test suite to pin the tables without a GPU).
  * lfmm_host_wigner_d: standard small Wigner matrix d^j(beta), row-major

@@ -31,7 +31,8 @@ STATUS = {0: "ok", 1: "invalid argument", 2: "expansion terms out of range", 3: 
 # Every extern "C" entry point declared in include/lfmm.h.
 EXPORTS = ("lfmm_create", "lfmm_destroy", "lfmm_strerror", "lfmm_set_stream", "lfmm_trim", "lfmm_fmm3d",
            "lfmm_tree3d", "lfmm_fmm2d", "lfmm_tree2d", "lfmm_version", "lfmm_host_wigner_d", "lfmm_host_op_tables",
-           "lfmm_host_m2l_dense", "lfmm_host_m2l16_pack")
+           "lfmm_host_m2l_dense", "lfmm_host_m2l16_pack", "lfmm_direct3d", "lfmm_direct2d", "lfmm_exact_filename",
+           "lfmm_write_exact", "lfmm_read_exact", "lfmm_error_stats")
 
 
 class LfmmError(RuntimeError):
@@ -118,6 +119,21 @@ def lib():
         L.lfmm_host_m2l16_pack.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                            ctypes.c_void_p]
         L.lfmm_host_m2l16_pack.restype = ctypes.c_int64
+        L.lfmm_direct3d.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points3d), ctypes.POINTER(Points3d), ctypes.c_int,
+                                    ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p]
+        L.lfmm_direct3d.restype = ctypes.c_int
+        L.lfmm_direct2d.argtypes = L.lfmm_direct3d.argtypes
+        L.lfmm_direct2d.restype = ctypes.c_int
+        L.lfmm_exact_filename.argtypes = [ctypes.c_int]
+        L.lfmm_exact_filename.restype = ctypes.c_char_p
+        L.lfmm_write_exact.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p]
+        L.lfmm_write_exact.restype = ctypes.c_int
+        L.lfmm_read_exact.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p]
+        L.lfmm_read_exact.restype = ctypes.c_int64
+        L.lfmm_error_stats.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double)]
+        L.lfmm_error_stats.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -239,6 +255,91 @@ def fmm2d(xy, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, excl
     Arguments as fmm3d with 2 coordinates per point."""
     return _solve(2, xy, q, eps, terms, tree_depth, leaf_max, exclude_self, timings, count_ops, ctx, out, targets,
                   False)
+
+
+# result kinds (the reference's fmm::InteractionType order)
+POTENTIAL2D, FORCE2D, POTENTIAL3D, FORCE3D = 0, 1, 2, 3
+_KIND_WIDTH = {POTENTIAL2D: 1, FORCE2D: 2, POTENTIAL3D: 1, FORCE3D: 3}
+
+
+def _direct(dim, xyz, q, targets, exclude_self, ctx):
+    ctx = ctx or default_context()
+    fn = lib().lfmm_direct3d if dim == 3 else lib().lfmm_direct2d
+    flags = FLAG_EXCLUDE_SELF if exclude_self else 0
+    if _is_torch_cuda(xyz):
+        import torch
+        if xyz.dtype != torch.float64 or q.dtype != torch.float64 or xyz.stride(-1) != 1:
+            raise TypeError("device inputs must be float64, contiguous per point")
+        pts = Points3d(xyz.shape[0], xyz.data_ptr(), xyz.stride(0), q.data_ptr(), q.stride(0))
+        tg, m = None, xyz.shape[0]
+        if targets is not None:
+            if targets.dtype != torch.float64 or targets.stride(-1) != 1:
+                raise TypeError("device targets must be float64, contiguous per point")
+            m = targets.shape[0]
+            tg = ctypes.byref(Points3d(m, targets.data_ptr(), targets.stride(0), None, 1))
+        pot = torch.empty(m, dtype=torch.float64, device=xyz.device)
+        field = torch.empty((m, dim), dtype=torch.float64, device=xyz.device)
+        _check(fn(ctx.handle, ctypes.byref(pts), tg, MEM_DEVICE, flags, ctypes.c_void_p(pot.data_ptr()),
+                  ctypes.c_void_p(field.data_ptr())), fn.__name__)
+        return pot, field
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, dim)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    pts = Points3d(len(q), xyz.ctypes.data, dim, q.ctypes.data, 1)
+    tg, m = None, len(q)
+    if targets is not None:
+        targets = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, dim)
+        m = len(targets)
+        tg = ctypes.byref(Points3d(m, targets.ctypes.data, dim, None, 1))
+    pot = np.empty(m)
+    field = np.empty((m, dim))
+    _check(fn(ctx.handle, ctypes.byref(pts), tg, MEM_HOST, flags, pot.ctypes.data, field.ctypes.data), fn.__name__)
+    return pot, field
+
+
+def direct3d(xyz, q, targets=None, exclude_self=False, ctx=None):
+    """FP64 direct sums on the GPU (reference fmm::gpu::ComputeExact with
+    Potential3d / Force3d): potential (m,) and field (m,3) at every target,
+    the self pair included unless exclude_self (single-set calls)."""
+    return _direct(3, xyz, q, targets, exclude_self, ctx)
+
+
+def direct2d(xy, q, targets=None, exclude_self=False, ctx=None):
+    """FP64 direct sums on the GPU (Potential2d / Force2d): potential (m,)
+    and the complex force (m,2) as (re, im)."""
+    return _direct(2, xy, q, targets, exclude_self, ctx)
+
+
+def exact_filename(kind):
+    """The reference's exact-result file name for a result kind."""
+    return lib().lfmm_exact_filename(int(kind)).decode()
+
+
+def write_exact(path, kind, values):
+    """Writes values in the reference's exact-result text format."""
+    v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1, _KIND_WIDTH[kind])
+    _check(lib().lfmm_write_exact(str(path).encode(), int(kind), len(v), v.ctypes.data), "lfmm_write_exact")
+
+
+def read_exact(path, kind, n):
+    """Reads n values of a reference exact-result file; returns (count read, values)."""
+    v = np.zeros((int(n), _KIND_WIDTH[kind]))
+    got = lib().lfmm_read_exact(str(path).encode(), int(kind), int(n), v.ctypes.data)
+    if got < 0:
+        raise LfmmError(int(-got), "lfmm_read_exact")
+    return int(got), (v[:, 0] if _KIND_WIDTH[kind] == 1 else v)
+
+
+def error_stats(kind, exact, result):
+    """The reference's ReadError figures: (max error, l2 error, relative error)."""
+    w = _KIND_WIDTH[kind]
+    a = np.ascontiguousarray(exact, dtype=np.float64).reshape(-1, w)
+    b = np.ascontiguousarray(result, dtype=np.float64).reshape(-1, w)
+    if a.shape != b.shape:
+        raise ValueError("shape mismatch")
+    mx, l2, rel = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    _check(lib().lfmm_error_stats(int(kind), len(a), a.ctypes.data, b.ctypes.data, ctypes.byref(mx), ctypes.byref(l2),
+                                  ctypes.byref(rel)), "lfmm_error_stats")
+    return mx.value, l2.value, rel.value
 
 
 def _tree(dim, xyz, q, tree_depth, leaf_max, ctx):

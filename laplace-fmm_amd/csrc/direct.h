@@ -1,0 +1,13 @@
+// direct.h -- FP64 all-pairs sums (direct.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace lfmm {
+
+// pot[i] (and field[dim * i ...]) over all sources for every target; packed
+// (x, y, z, q) records (2D: (x, y, 0, q)).  exclude_self skips j == i.
+hipError_t launch_direct(int dim, const double4* tgt, int64_t nt, const double4* src, int64_t ns, bool exclude_self,
+                         double* pot, double* field, hipStream_t st);
+
+}  // namespace lfmm

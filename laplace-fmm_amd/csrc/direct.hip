@@ -1,0 +1,118 @@
+// direct.hip -- FP64 all-pairs sums on the GPU (the reference's
+// fmm::gpu::ComputeExact, src/common/simple_math.cu:9-22 and :24-130, with the
+// pair terms of src/common/simple_math.h:120-151 and their device forms
+// :197-208).
+//
+// One thread per target; sources stream through LDS in tiles of 256 packed
+// (x, y, z, q) records, each added in input order.  3D pair:
+//   q / max(|r|, 1e-4)  and  q r / max(|r|, 1e-4)^3   (r = x_i - x_j)
+// evaluated as q * rsqrt(max(|r|^2, 1e-8)), the reference's device form.
+// 2D pair: q * 0.5 log max(|dz|^2, 1e-8) and q conj(dz) / max(|dz|^2, 1e-8).
+// With `exclude` the pair j == i is skipped (single-set calls only).
+#include "device_common.h"
+#include "direct.h"
+
+namespace lfmm {
+
+namespace {
+
+constexpr int kTile = 256;
+
+// 1 / sqrt(r2): v_rsq_f64 refined by two Newton steps (full FP64 accuracy)
+__device__ __forceinline__ double rsqrt_full(double r2)
+{
+    double y = __builtin_amdgcn_rsq(r2);
+    const double hr = 0.5 * r2;
+    y = y * fma(-hr * y, y, 1.5);
+    y = y * fma(-hr * y, y, 1.5);
+    return y;
+}
+
+__global__ void __launch_bounds__(kTile) k_direct3d(const double4* __restrict__ tgt, int64_t nt,
+                                                   const double4* __restrict__ src, int64_t ns, int exclude,
+                                                   double* __restrict__ pot, double* __restrict__ field)
+{
+    __shared__ double4 tile[kTile];
+    const int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    const double4 t = i < nt ? tgt[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    double p = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int64_t base = 0; base < ns; base += kTile) {
+        __syncthreads();
+        const int64_t j0 = base + threadIdx.x;
+        tile[threadIdx.x] = j0 < ns ? src[j0] : make_double4(0.0, 0.0, 0.0, 0.0);
+        __syncthreads();
+        const int cnt = (int)min<int64_t>(kTile, ns - base);
+        for (int j = 0; j < cnt; ++j) {
+            if (exclude && base + j == i) continue;
+            const double4 s = tile[j];
+            const double dx = t.x - s.x, dy = t.y - s.y, dz = t.z - s.z;
+            const double r2 = fmax(fma(dx, dx, fma(dy, dy, dz * dz)), 1.e-8);
+            const double ir = rsqrt_full(r2);
+            const double qi = s.w * ir;
+            p += qi;
+            const double q3 = qi * ir * ir;
+            fx = fma(q3, dx, fx);
+            fy = fma(q3, dy, fy);
+            fz = fma(q3, dz, fz);
+        }
+    }
+    if (i < nt) {
+        if (pot) pot[i] = p;
+        if (field) {
+            field[3 * i] = fx;
+            field[3 * i + 1] = fy;
+            field[3 * i + 2] = fz;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kTile) k_direct2d(const double4* __restrict__ tgt, int64_t nt,
+                                                   const double4* __restrict__ src, int64_t ns, int exclude,
+                                                   double* __restrict__ pot, double* __restrict__ field)
+{
+    __shared__ double4 tile[kTile];
+    const int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    const double4 t = i < nt ? tgt[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    double p = 0.0, fx = 0.0, fy = 0.0;
+    for (int64_t base = 0; base < ns; base += kTile) {
+        __syncthreads();
+        const int64_t j0 = base + threadIdx.x;
+        tile[threadIdx.x] = j0 < ns ? src[j0] : make_double4(0.0, 0.0, 0.0, 0.0);
+        __syncthreads();
+        const int cnt = (int)min<int64_t>(kTile, ns - base);
+        for (int j = 0; j < cnt; ++j) {
+            if (exclude && base + j == i) continue;
+            const double4 s = tile[j];
+            // packed 2D records are (x, y, 0, q)
+            const double dx = t.x - s.x, dy = t.y - s.y;
+            const double r2 = fmax(fma(dx, dx, dy * dy), 1.e-8);
+            p = fma(s.w * 0.5, log(r2), p);
+            const double qr = s.w / r2;
+            fx = fma(qr, dx, fx);
+            fy = fma(-qr, dy, fy);
+        }
+    }
+    if (i < nt) {
+        if (pot) pot[i] = p;
+        if (field) {
+            field[2 * i] = fx;
+            field[2 * i + 1] = fy;
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_direct(int dim, const double4* tgt, int64_t nt, const double4* src, int64_t ns, bool exclude_self,
+                         double* pot, double* field, hipStream_t st)
+{
+    if (nt <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((nt + kTile - 1) / kTile));
+    if (dim == 3)
+        hipLaunchKernelGGL(k_direct3d, grid, dim3(kTile), 0, st, tgt, nt, src, ns, exclude_self ? 1 : 0, pot, field);
+    else
+        hipLaunchKernelGGL(k_direct2d, grid, dim3(kTile), 0, st, tgt, nt, src, ns, exclude_self ? 1 : 0, pot, field);
+    return hipGetLastError();
+}
+
+}  // namespace lfmm

@@ -45,6 +45,8 @@ def test_invalid_arguments_are_rejected_without_gpu(lfmm):
     assert L.lfmm_fmm3d(None, ctypes.byref(pts), None, 1e-8, 5, 3, 0, 0, 0, None, None, None) == 1
     assert L.lfmm_set_stream(None, None) == 1
     assert L.lfmm_tree3d(None, None, 3, 0, None, None, None, None, None, None) == 1
+    assert L.lfmm_direct3d(None, ctypes.byref(pts), None, 0, 0, None, None) == 1
+    assert L.lfmm_direct2d(None, ctypes.byref(pts), None, 0, 0, None, None) == 1
 
 
 def test_cpp_dropin_header_compiles_and_links(lfmm, tmp_path):
