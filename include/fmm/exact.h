@@ -40,7 +40,7 @@ lfmm_points3d exact_view(const std::vector<particle<point_type, value_type>>& p)
     v.n = (int64_t)p.size();
     v.pos = p.empty() ? nullptr : reinterpret_cast<const double*>(&p[0].center);
     v.pos_stride = (int64_t)(sizeof(particle<point_type, value_type>) / sizeof(double));
-    v.q = p.empty() ? nullptr : &p[0].q;
+    v.q = p.empty() ? nullptr : reinterpret_cast<const double*>(&p[0].q);
     v.q_stride = v.pos_stride;
     return v;
 }
@@ -55,12 +55,23 @@ void ComputeExact(const std::vector<particle<point_type, value_type>>& source_pa
 {
     constexpr bool two_d = it == InteractionType::Potential2d || it == InteractionType::Force2d;
     constexpr bool potential = it == InteractionType::Potential2d || it == InteractionType::Potential3d;
-    static_assert(std::is_same_v<value_type, double>, "scalar charges");
     static_assert(two_d == std::is_same_v<point_type, std::complex<double>>, "2D kinds take complex positions");
     const lfmm_points3d s = detail::exact_view(source_particles);
     const lfmm_points3d t = detail::exact_view(target_particles);
     const bool same = &source_particles == &target_particles;
     const size_t n = target_particles.size();
+    if constexpr (std::is_same_v<value_type, Vector3d>) {
+        // vector charges: the force q x r / |r|^3 only (simple_math.h:148-152)
+        static_assert(it == InteractionType::Force3d, "vector charges have forces only");
+        std::vector<double> f(3 * n);
+        detail::exact_check(lfmm_direct3d_vec(detail::thread_context(), &s, same ? nullptr : &t, LFMM_MEM_HOST,
+                                              f.data()),
+                            "lfmm_direct3d_vec");
+        const int kind = (int)it;
+        detail::exact_check(lfmm_write_exact(lfmm_exact_filename(kind), kind, (int64_t)n, f.data()), "lfmm_write_exact");
+        return;
+    }
+    static_assert(std::is_same_v<value_type, double> || std::is_same_v<value_type, Vector3d>, "charges");
     const int width = potential ? 1 : (two_d ? 2 : 3);
     std::vector<double> values(n * width);
     double* pot = potential ? values.data() : nullptr;

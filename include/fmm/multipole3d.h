@@ -5,6 +5,7 @@
 //   fmm::FMM_AUTO, fmm::Vector3d, fmm::point3d, fmm::particle<point, value>,
 //   fmm::particle3d                           (src/common/defs.h:22, utils.h:26-112)
 //   fmm::gpu::FastMultipole3d<double>         (src/cuda/multipole3d.h:7-42)
+//   fmm::gpu::FastMultipole3d<Vector3d>, fmm::particle3d3   vector charges (forces only)
 //     FastMultipole3d(particles, eps = 1e-8, N = FMM_AUTO, tree_depth = FMM_AUTO)
 //     FastMultipole3d(sources, targets, eps, N, tree_depth)
 //     std::vector<Vector3d> forces;  std::vector<double> potentials;
@@ -140,7 +141,69 @@ private:
     }
 };
 
+// Vector charges (the reference's FastMultipole3d<Vector3d>): forces
+// F_i = sum_j q_j x (r_i - r_j) / |r_i - r_j|^3; `potentials` stays empty, as
+// in the reference.  Particles are the reference's 56-byte particle3d3.
+template <>
+class FastMultipole3d<Vector3d> {
+public:
+    using particle_t = fmm::particle<point3d, Vector3d>;
+
+    FastMultipole3d(const std::vector<particle_t>& particles, double eps = 1.e-8, int N = FMM_AUTO,
+                    int tree_depth = FMM_AUTO)
+    {
+        Solve(particles, nullptr, eps, N, tree_depth);
+    }
+
+    FastMultipole3d(const std::vector<particle_t>& source_particles, const std::vector<particle_t>& target_particles,
+                    double eps = 1.e-8, int N = FMM_AUTO, int tree_depth = FMM_AUTO)
+    {
+        Solve(source_particles, &target_particles, eps, N, tree_depth);
+    }
+
+    std::vector<Vector3d> forces;
+    std::vector<double> potentials;
+
+private:
+    static_assert(sizeof(particle_t) == 7 * sizeof(double), "reference particle3d3 layout: 56 bytes");
+
+    static lfmm_points3d view(const std::vector<particle_t>& p)
+    {
+        lfmm_points3d v{};
+        v.n = (int64_t)p.size();
+        v.pos = p.empty() ? nullptr : &p[0].center[0];
+        v.pos_stride = 7;
+        v.q = p.empty() ? nullptr : &p[0].q[0];
+        v.q_stride = 7;
+        return v;
+    }
+
+    void Solve(const std::vector<particle_t>& src, const std::vector<particle_t>* tgt, double eps, int N, int depth)
+    {
+        const lfmm_points3d s = view(src);
+        lfmm_points3d t{};
+        if (tgt) t = view(*tgt);
+        const size_t n_out = tgt ? tgt->size() : src.size();
+        forces.assign(n_out, Vector3d());
+        const int rc = lfmm_fmm3d_vec(detail::thread_context(), &s, tgt ? &t : nullptr, eps, N, depth, 0,
+                                      LFMM_MEM_HOST, 0u, forces.empty() ? nullptr : &forces[0][0], nullptr);
+        if (rc != LFMM_OK) throw std::runtime_error(std::string("lfmm_fmm3d_vec: ") + lfmm_strerror(rc));
+    }
+};
+
+// deduction from the particle type, as the reference's drivers construct
+// (src/examples/main.cpp:190, 241)
+template <typename value>
+FastMultipole3d(const std::vector<fmm::particle<point3d, value>>&, double = 1.e-8, int = FMM_AUTO, int = FMM_AUTO)
+    -> FastMultipole3d<value>;
+template <typename value>
+FastMultipole3d(const std::vector<fmm::particle<point3d, value>>&, const std::vector<fmm::particle<point3d, value>>&,
+                double = 1.e-8, int = FMM_AUTO, int = FMM_AUTO) -> FastMultipole3d<value>;
+
 }  // namespace gpu
+
+using particle3d3 = particle<point3d, Vector3d>;
+
 }  // namespace fmm
 
 // the checking path (direct sums, exact-result files, ReadError)

@@ -199,6 +199,29 @@ int lfmm_direct3d(lfmm_ctx* ctx, const lfmm_points3d* sources, const lfmm_points
 int lfmm_direct2d(lfmm_ctx* ctx, const lfmm_points2d* sources, const lfmm_points2d* targets, int memory,
                   unsigned flags, double* potential, double* field);
 
+/*
+ * Vector charges: replaces fmm::gpu::FastMultipole3d<Vector3d>
+ * (src/cuda/multipole3d.h:7-42 with value = Vector3d; kernels
+ * src/cuda/multipole3d.cu:633-839, CPU twin src/cpu/multipole3d.cpp:414-853).
+ * Each particle carries a charge vector (q[i*q_stride], q[i*q_stride+1],
+ * q[i*q_stride+2]); the reference's 56-byte particle3d3 {Vector3d center;
+ * Vector3d q; size_t morton} uses pos = &p[0].center[0], q = &p[0].q[0] and
+ * strides 7 and 7.  Output: force n_out*3 doubles,
+ *   F_i = sum_j q_j x (r_i - r_j) / max(|r_i - r_j|, 1e-4)^3
+ * (src/common/simple_math.h:148-152); no potentials, as in the reference.
+ * Computed as three scalar solves (one per charge component, the same tree)
+ * whose fields are combined by a cross product, as the reference's
+ * forces_cross (src/cuda/multipole3d.cu:765-776).  Other arguments as
+ * lfmm_fmm3d (the report sums the timings of the three solves).
+ * lfmm_direct3d_vec: the same sums directly, in FP64 (reference
+ * ComputeExact<InteractionType::Force3d> on particle3d3).
+ */
+int lfmm_fmm3d_vec(lfmm_ctx* ctx, const lfmm_points3d* sources, const lfmm_points3d* targets,
+                   double eps, int terms, int tree_depth, int leaf_max,
+                   int memory, unsigned flags, double* force, lfmm_report* report);
+int lfmm_direct3d_vec(lfmm_ctx* ctx, const lfmm_points3d* sources, const lfmm_points3d* targets, int memory,
+                      double* force);
+
 /* Result kinds, in the order of the reference's fmm::InteractionType
  * (src/common/utils.h:19-24). */
 typedef enum lfmm_interaction {

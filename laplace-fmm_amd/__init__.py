@@ -32,7 +32,7 @@ STATUS = {0: "ok", 1: "invalid argument", 2: "expansion terms out of range", 3: 
 EXPORTS = ("lfmm_create", "lfmm_destroy", "lfmm_strerror", "lfmm_set_stream", "lfmm_trim", "lfmm_fmm3d",
            "lfmm_tree3d", "lfmm_fmm2d", "lfmm_tree2d", "lfmm_version", "lfmm_host_wigner_d", "lfmm_host_op_tables",
            "lfmm_host_m2l_dense", "lfmm_host_m2l16_pack", "lfmm_direct3d", "lfmm_direct2d", "lfmm_exact_filename",
-           "lfmm_write_exact", "lfmm_read_exact", "lfmm_error_stats")
+           "lfmm_write_exact", "lfmm_read_exact", "lfmm_error_stats", "lfmm_fmm3d_vec", "lfmm_direct3d_vec")
 
 
 class LfmmError(RuntimeError):
@@ -134,6 +134,13 @@ def lib():
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_double)]
         L.lfmm_error_stats.restype = ctypes.c_int
+        L.lfmm_fmm3d_vec.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points3d), ctypes.POINTER(Points3d),
+                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_uint, ctypes.c_void_p, ctypes.POINTER(Report)]
+        L.lfmm_fmm3d_vec.restype = ctypes.c_int
+        L.lfmm_direct3d_vec.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points3d), ctypes.POINTER(Points3d),
+                                        ctypes.c_int, ctypes.c_void_p]
+        L.lfmm_direct3d_vec.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -294,6 +301,51 @@ def _direct(dim, xyz, q, targets, exclude_self, ctx):
     field = np.empty((m, dim))
     _check(fn(ctx.handle, ctypes.byref(pts), tg, MEM_HOST, flags, pot.ctypes.data, field.ctypes.data), fn.__name__)
     return pot, field
+
+
+def _vec_call(fn, name, xyz, qvec, targets, ctx, extra):
+    ctx = ctx or default_context()
+    if _is_torch_cuda(xyz):
+        import torch
+        if xyz.dtype != torch.float64 or qvec.dtype != torch.float64 or xyz.stride(-1) != 1 or qvec.stride(-1) != 1:
+            raise TypeError("device inputs must be float64, contiguous per point")
+        pts = Points3d(xyz.shape[0], xyz.data_ptr(), xyz.stride(0), qvec.data_ptr(), qvec.stride(0))
+        tg, m = None, xyz.shape[0]
+        if targets is not None:
+            m = targets.shape[0]
+            tg = ctypes.byref(Points3d(m, targets.data_ptr(), targets.stride(0), None, 1))
+        force = torch.empty((m, 3), dtype=torch.float64, device=xyz.device)
+        _check(fn(ctx.handle, ctypes.byref(pts), tg, *extra(MEM_DEVICE, ctypes.c_void_p(force.data_ptr()))), name)
+        return force
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    qvec = np.ascontiguousarray(qvec, dtype=np.float64).reshape(-1, 3)
+    pts = Points3d(len(qvec), xyz.ctypes.data, 3, qvec.ctypes.data, 3)
+    tg, m = None, len(qvec)
+    if targets is not None:
+        targets = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+        m = len(targets)
+        tg = ctypes.byref(Points3d(m, targets.ctypes.data, 3, None, 1))
+    force = np.empty((m, 3))
+    _check(fn(ctx.handle, ctypes.byref(pts), tg, *extra(MEM_HOST, force.ctypes.data)), name)
+    return force
+
+
+def fmm3d_vec(xyz, qvec, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, ctx=None, targets=None,
+              mixed=False):
+    """FMM with vector charges (reference FastMultipole3d<Vector3d>): the
+    force sum q_j x (r_i - r_j) / |r_i - r_j|^3 (m,3) at every target."""
+    rep = Report()
+    flags = FLAG_MIXED if mixed else 0
+    force = _vec_call(lib().lfmm_fmm3d_vec, "lfmm_fmm3d_vec", xyz, qvec, targets, ctx,
+                      lambda mem, out: (float(eps), int(terms), int(tree_depth), int(leaf_max), mem, flags, out,
+                                        ctypes.byref(rep)))
+    return force, rep.as_dict()
+
+
+def direct3d_vec(xyz, qvec, targets=None, ctx=None):
+    """FP64 direct sums of the vector-charge force (reference ComputeExact on particle3d3)."""
+    return _vec_call(lib().lfmm_direct3d_vec, "lfmm_direct3d_vec", xyz, qvec, targets, ctx,
+                     lambda mem, out: (mem, out))
 
 
 def direct3d(xyz, q, targets=None, exclude_self=False, ctx=None):

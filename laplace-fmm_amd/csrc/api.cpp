@@ -354,10 +354,13 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     if (mixed) {
         rc = get_m16(ctx, N, &DT);
         if (rc != LFMM_OK) return rc;
+        // packed sources of the largest mixed level: box-major (size + 1
+        // boxes) or class-major (8 classes x (parents + 1) slots)
         int64_t big = 0;
-        for (int l = kMixedFirstLevel; l < depth; ++l) big = std::max(big, cnt[l]);
+        for (int l = kMixedFirstLevel; l < depth; ++l)
+            big = std::max(big, m16_rowfull(N) ? 8 * (cnt[l - 1] + 1) : cnt[l] + 1);
         const int64_t nr = (int64_t)N * N, pieces = (nr + 15) / 16 * 4;
-        xp = take<uint4>(A, (big + 1) * pieces);
+        xp = take<uint4>(A, big * pieces);
         xmax = take<uint32_t>(A, depth);
         if (!xp || !xmax) return LFMM_ERR_NOMEM;
         LFMM_HIP(fill_u32(xmax, depth, 0u, st));
@@ -423,6 +426,8 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                 m.child_keys = L.child_keys;
                 m.lut = L.lut;
                 m.keys = L.keys;
+                m.plut = lut[l - 1];
+                m.pkeys = lkeys[l - 1];
                 m.size = L.size;
                 m.kinv = DT->m16_kinv;
                 m.dst = inner[l];
@@ -497,6 +502,10 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     la.exclude_self = (flags & LFMM_FLAG_EXCLUDE_SELF) ? 1 : 0;
     la.pot = dpot;
     la.field = dfield;
+    if (far) {  // L2P and the pair loop as two kernels (see k_l2p)
+        la.far = take<double4>(A, n);
+        if (!la.far) return LFMM_ERR_NOMEM;
+    }
     const size_t ev_leaf = nev;
     if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
     LFMM_HIP(launch_leaf(N, far, la, st));

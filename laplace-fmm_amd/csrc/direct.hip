@@ -101,7 +101,34 @@ __global__ void __launch_bounds__(kTile) k_direct2d(const double4* __restrict__ 
     }
 }
 
+// Field of vector charges from the fields of their components: with
+// E_c = sum_j q_j^c r / |r|^3, sum_j q_j x r / |r|^3 = (E_2.z - E_3.y,
+// E_3.x - E_1.z, E_1.y - E_2.x) (the reference's forces_cross,
+// src/cuda/multipole3d.cu:765-776).  Written over e3.
+__global__ void __launch_bounds__(256) k_cross_fields(const double* __restrict__ e1, const double* __restrict__ e2,
+                                                     double* __restrict__ e3, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a0 = e1[3 * i], a1 = e1[3 * i + 1], a2 = e1[3 * i + 2];
+    const double b0 = e2[3 * i], b1 = e2[3 * i + 1], b2 = e2[3 * i + 2];
+    const double c0 = e3[3 * i], c1 = e3[3 * i + 1], c2 = e3[3 * i + 2];
+    (void)a0;
+    (void)b1;
+    (void)c2;
+    e3[3 * i] = b2 - c1;
+    e3[3 * i + 1] = c0 - a2;
+    e3[3 * i + 2] = a1 - b0;
+}
+
 }  // namespace
+
+hipError_t launch_cross_fields(const double* e1, const double* e2, double* e3, int64_t n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cross_fields, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, e1, e2, e3, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_direct(int dim, const double4* tgt, int64_t nt, const double4* src, int64_t ns, bool exclude_self,
                          double* pot, double* field, hipStream_t st)

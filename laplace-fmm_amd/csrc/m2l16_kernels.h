@@ -91,6 +91,52 @@ __global__ void __launch_bounds__(256) k_m2l16_pack(const double2* __restrict__ 
     dst[2 + half] = __builtin_bit_cast(uint4, lv);
 }
 
+// Packs a level's normalised multipoles class-major for k_m2l16r:
+// piece (kc, hi/lo, half) of the class-c child of parent P lands at
+// xp[(c * PIECES + 4 kc + 2 hl + half) * (num_parents + 1) + P], zeros where
+// the child is absent and in slot num_parents.  For one direction, the
+// sources of a tile's targets are one class and a shifted block of parents,
+// so a source fragment reads runs of consecutive 16-byte entries.
+template <int N>
+__global__ void __launch_bounds__(256) k_m2l16_pack_cls(const double2* __restrict__ outer, int num_parents,
+                                                        const int32_t* __restrict__ child_begin,
+                                                        const uint64_t* __restrict__ child_keys,
+                                                        const uint32_t* __restrict__ xmax, uint4* __restrict__ xp)
+{
+    using S = M16<N>;
+    const int64_t np1 = (int64_t)num_parents + 1;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (kc, half, parent), parent fastest
+    if (i >= np1 * S::KC * 2) return;
+    const int64_t P = i % np1;
+    const int kh = (int)(i / np1);
+    const int kc = kh >> 1, half = kh & 1;
+    const double sc = m16_xscale(*xmax);
+    int j = 0, ce = 0;
+    if (P < num_parents) {
+        j = child_begin[P];
+        ce = child_begin[P + 1];
+    }
+    for (int c = 0; c < 8; ++c) {
+        // children are in key order, so classes come in increasing order
+        const int child = (j < ce && (int)(child_keys[j] & 7u) == c) ? j++ : -1;
+        m16_h8 hv = {}, lv = {};
+        if (child >= 0) {
+            const double* src = (const double*)(outer + (int64_t)child * tri(N));
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int k = kc * 16 + half * 8 + s;
+                const double x = k < S::NR ? src[m16_coef_offset(k)] * sc : 0.0;
+                const _Float16 h = (_Float16)x;
+                hv[s] = h;
+                lv[s] = (_Float16)(x - (double)h);
+            }
+        }
+        uint4* plane = xp + (int64_t)(c * S::PIECES + kc * 4 + half) * np1;
+        plane[P] = __builtin_bit_cast(uint4, hv);
+        plane[2 * np1 + P] = __builtin_bit_cast(uint4, lv);
+    }
+}
+
 // Block skipping (the prefix table) pays where the operator blocks are many
 // and sparse; below 12 terms the straight-line MFMA stream is faster.
 constexpr bool m16_skip(int N) { return N >= 12; }
@@ -367,7 +413,7 @@ struct M16R {
 constexpr int m16r_ctw(int N) { return m16r_max_rt2 >= (N * N + 31) / 32 ? 2 : 1; }
 
 template <int N, int CTW>
-__global__ void __launch_bounds__(256) k_m2l16r(M2L16Args a, const uint4* __restrict__ kmat, const uint4* __restrict__ xp,
+__global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, const uint4* __restrict__ kmat, const uint4* __restrict__ xp,
                                                 const int* __restrict__ class_dirs, const int* __restrict__ dir_vec,
                                                 const int2* __restrict__ kpre)
 {
@@ -414,18 +460,21 @@ __global__ void __launch_bounds__(256) k_m2l16r(M2L16Args a, const uint4* __rest
         }
     }
     auto dir_of = [&](int d) { return class_dirs[c * kDirsPerClass + d]; };
-    // source box of column tile ct in direction dir (the zero box when absent)
-    auto box_of = [&](int dir, int ct) -> int {
-        int box = a.size;
+    // source of column tile ct in direction dir, as the offset of its class
+    // plane entry: class * PIECES * (num_parents + 1) + parent slot (the zero
+    // slot num_parents of class 0 when the source or its parent is absent)
+    const int64_t np1 = (int64_t)a.num_parents + 1;
+    auto box_of = [&](int dir, int ct) -> int64_t {
+        int64_t base = a.num_parents;
         if (tgt[ct] >= 0) {
             const int sx = (int)tx[ct] + dir_vec[4 * dir], sy = (int)ty[ct] + dir_vec[4 * dir + 1],
                       sz = (int)tz[ct] + dir_vec[4 * dir + 2];
             if (sx >= 0 && sy >= 0 && sz >= 0 && sx < side && sy < side && sz < side) {
-                const int s = find_box(morton3(sx, sy, sz), a.lut, a.keys, a.size);
-                if (s >= 0) box = s;
+                const int p = find_box(morton3(sx >> 1, sy >> 1, sz >> 1), a.plut, a.pkeys, a.num_parents);
+                if (p >= 0) base = (int64_t)(((sx & 1) << 2) | ((sy & 1) << 1) | (sz & 1)) * PIECES * np1 + p;
             }
         }
-        return box;
+        return base;
     };
     // bounds of direction dir (wave-uniform): K chunks below k1 need hi*hi,
     // below k3 all three products, for some row tile
@@ -470,22 +519,26 @@ __global__ void __launch_bounds__(256) k_m2l16r(M2L16Args a, const uint4* __rest
     // source fragments of chunk kc (both column tiles): hi below the
     // direction's bound k1 (else the zero box), lo only below its bound k3
     // (the lo products are not issued beyond it)
-    auto load_b = [&](const int* box, int kc, int bk3, int bk1, m16_u4 (&b)[CTW][2]) {
+    auto load_b = [&](const int64_t* box, int kc, int bk3, int bk1, m16_u4 (&b)[CTW][2]) {
+        const int64_t ph = (int64_t)(kc * 4 + (lane >> 5)) * np1;
 #pragma unroll
         for (int ct = 0; ct < CTW; ++ct) {
-            const int bh = kc < bk1 ? box[ct] : a.size;
-            b[ct][0] = xv[(int64_t)bh * PIECES + kc * 4 + (lane >> 5)];
+            const int64_t bh = kc < bk1 ? box[ct] : (int64_t)a.num_parents;
+            b[ct][0] = xv[bh + ph];
         }
         if (kc < bk3) {
 #pragma unroll
-            for (int ct = 0; ct < CTW; ++ct) b[ct][1] = xv[(int64_t)box[ct] * PIECES + kc * 4 + 2 + (lane >> 5)];
+            for (int ct = 0; ct < CTW; ++ct) b[ct][1] = xv[box[ct] + ph + 2 * np1];
         }
     };
     // steps e = s * SK + kci of a direction in execution order (chunk
-    // kc = s + NSEG * kci, absent when kc >= KC); NSTEP is a multiple of 3,
-    // so a 3-slot ring of source fragments keeps its slots across directions
+    // kc = s + NSEG * kci, absent when kc >= KC).  The ring of source
+    // fragments has RB slots (loads LB = RB - 1 steps ahead); RB divides
+    // NSTEP, so the ring keeps its slots across directions.
     constexpr int NSTEP = NSEG * SK;
-    static_assert(NSTEP % 3 == 0, "source ring");
+    constexpr int RB = NSTEP % 3 == 0 ? 3 : (NSTEP % 4 == 0 ? 4 : 2);
+    constexpr int LB = RB - 1;
+    static_assert(NSTEP % RB == 0, "source ring");
     auto kc_of = [](int e) { return e / SK + NSEG * (e % SK); };
 
     // accumulators start as the product of zero operands, so they are
@@ -503,28 +556,42 @@ __global__ void __launch_bounds__(256) k_m2l16r(M2L16Args a, const uint4* __rest
     int dcur = dir_of(d_beg);
     int bk3, bk1;
     prefixes(dcur, bk3, bk1);
-    int bx[CTW];
+    int64_t bx[CTW];
 #pragma unroll
     for (int ct = 0; ct < CTW; ++ct) bx[ct] = box_of(dcur, ct);
     Stage stg;
-    m16_u4 br[3][CTW][2];  // source fragments of steps e, e + 1, e + 2
+    m16_u4 br[RB][CTW][2];  // source fragments of steps e .. e + LB
     m16_u4* cur = &as[0][0][0];
     m16_u4* nxt = &as[1][0][0];
     stg = stage_a(dcur, 0, bk3, bk1);
 #pragma unroll
-    for (int e = 0; e < 2; ++e)
+    for (int e = 0; e < LB; ++e)
         if (kc_of(e) < KC) load_b(bx, kc_of(e), bk3, bk1, br[e]);
     store_a(cur, stg);
     lds_barrier();
     for (int d = d_beg; d < d_end; ++d) {
         const int dnext = dir_of(min(d + 1, d_end - 1));  // the last step re-stages its own data (never read)
-        int bxn[CTW];
+        int64_t bxn[CTW];
 #pragma unroll
         for (int ct = 0; ct < CTW; ++ct) bxn[ct] = box_of(dnext, ct);
         int bk3n, bk1n;
         prefixes(dnext, bk3n, bk1n);
+        // sources of step e + LB (into ring slot (e + LB) % RB)
+        auto issue_b = [&](int e) {
+            const int e2 = e + LB;
+            if (e2 < NSTEP) {
+                if (kc_of(e2) < KC) load_b(bx, kc_of(e2), bk3, bk1, br[e2 % RB]);
+            } else if (kc_of(e2 - NSTEP) < KC) {
+                load_b(bxn, kc_of(e2 - NSTEP), bk3n, bk1n, br[e2 % RB]);
+            }
+        };
 #pragma unroll
         for (int s = 0; s < NSEG; ++s) {
+            // The first source load of the segment goes out before the
+            // operator staging: vector loads complete in issue order, so the
+            // wait for it (two steps later) does not also wait for the
+            // operator fragments, which come from further away.
+            issue_b(s * SK);
             // the next segment's operator fragments, in flight under this
             // segment's MFMAs (written to the other LDS buffer at its end)
             if (s + 1 < NSEG) stg = stage_a(dcur, s + 1, bk3, bk1);
@@ -540,16 +607,9 @@ __global__ void __launch_bounds__(256) k_m2l16r(M2L16Args a, const uint4* __rest
 #pragma unroll
             for (int kci = 0; kci < SK; ++kci) {
                 const int e = s * SK + kci, kc = kc_of(e);
-                {
-                    const int e2 = e + 2;
-                    if (e2 < NSTEP) {
-                        if (kc_of(e2) < KC) load_b(bx, kc_of(e2), bk3, bk1, br[e2 % 3]);
-                    } else if (kc_of(e2 - NSTEP) < KC) {
-                        load_b(bxn, kc_of(e2 - NSTEP), bk3n, bk1n, br[e2 % 3]);
-                    }
-                }
+                if (kci > 0) issue_b(e);
                 if (kc < KC) {
-                    const m16_u4(&b)[CTW][2] = br[e % 3];
+                    const m16_u4(&b)[CTW][2] = br[e % RB];
                     const int h = kci & 1;
                     m16_u4 fl[RT];
 #pragma unroll
@@ -633,9 +693,15 @@ constexpr int m16_ct(int N) { return N <= 16 ? 2 : 1; }
 template <int N>
 hipError_t do_m2l16_pack(const M2L16Args& a, hipStream_t st)
 {
-    const int64_t packs = (int64_t)(a.size + 1) * M16<N>::KC * 2;
-    hipLaunchKernelGGL((k_m2l16_pack<N>), dim3((unsigned)((packs + 255) / 256)), dim3(256), 0, st, a.outer, a.size,
-                       a.xmax, a.xp);
+    if constexpr (m16_rowfull(N)) {
+        const int64_t packs = ((int64_t)a.num_parents + 1) * M16<N>::KC * 2;
+        hipLaunchKernelGGL((k_m2l16_pack_cls<N>), dim3((unsigned)((packs + 255) / 256)), dim3(256), 0, st, a.outer,
+                           a.num_parents, a.child_begin, a.child_keys, a.xmax, a.xp);
+    } else {
+        const int64_t packs = (int64_t)(a.size + 1) * M16<N>::KC * 2;
+        hipLaunchKernelGGL((k_m2l16_pack<N>), dim3((unsigned)((packs + 255) / 256)), dim3(256), 0, st, a.outer,
+                           a.size, a.xmax, a.xp);
+    }
     return hipGetLastError();
 }
 

@@ -73,6 +73,8 @@ struct LeafArgs {
     int exclude_self;
     double* pot;
     double* field;
+    double4* far = nullptr;    // with locals: per sorted particle (phi, field) of L2P, written by
+                               // k_l2p and read by the pair kernel (split launch)
 };
 
 // Matrix-core M2L of one level (m2l16_kernels.h).
@@ -80,7 +82,8 @@ struct M2L16Args {
     const uint4* kmat;         // split-fp16 operator fragments [dir][RT][KC][2][64]
     const int2* kpre;          // [dir][RT] (k3, k1): K chunks needing all products / hi*hi
     const double2* outer;      // normalised multipoles of the level
-    uint4* xp;                 // packed sources [size + 1][KC][2][2] (box `size` is zero)
+    uint4* xp;                 // packed sources: k_m2l16 [size + 1][KC][2][2] (box `size` is zero);
+                               // k_m2l16r [8 classes][KC][2][2][num_parents + 1] (slot num_parents is zero)
     const uint32_t* xmax;      // largest |coefficient| of the level (float bits)
     const int* class_dirs;
     const int* dir_vec;
@@ -90,6 +93,8 @@ struct M2L16Args {
     const uint64_t* child_keys;
     const int32_t* lut;        // dense key -> box table at `level`, or null
     const uint64_t* keys;
+    const int32_t* plut;       // the same at level - 1 (parents), or null
+    const uint64_t* pkeys;
     int size;                  // boxes at `level`
     double kinv;               // 1 / scale of the packed operators
     double2* dst;              // locals of the level (+= M2L)
