@@ -63,6 +63,21 @@ def m2l16_mfma_flops_per_op(lfmm, N):
     return float(per_dir[cd].mean())
 
 
+def plummer(n, seed, a=0.05, rmax=1.0):
+    """Plummer sphere of scale radius a truncated at rmax, centred at
+    (1/2, 1/2, 1/2): radii r = a / sqrt(u^(-2/3) - 1) (redrawn beyond rmax),
+    isotropic directions; charges U[-1,1]."""
+    rng = np.random.default_rng(seed)
+    r = np.empty(0)
+    while len(r) < n:
+        u = rng.random(2 * n)
+        rr = a / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+        r = np.concatenate([r, rr[rr <= rmax]])
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    return 0.5 + r[:n, None] * d, rng.uniform(-1.0, 1.0, n)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +193,30 @@ def main():
             r = run(terms, mix, 3, 1)
             extras.append({"terms": terms, "m2l": "split-FP16 MFMA (levels >= 4)" if mix else "FP64", "r": r})
 
+    # Plummer sphere (the north star's distribution): fixed order, uniform
+    # depth from leaf_max 64, FP64 against the matrix-core path
+    plum = []
+    if not args.no_extra:
+        xp_, qp_ = plummer(args.n, args.seed + 11)
+        xpd, qpd = torch.from_numpy(xp_).to(dev), torch.from_numpy(qp_).to(dev)
+        for mix in (False, True):
+            def pstep():
+                return lfmm.fmm3d(xpd, qpd, terms=args.terms, leaf_max=64, ctx=ctx, out=(pot, field), mixed=mix)[2]
+            rep = pstep()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                pstep()
+            torch.cuda.synchronize()
+            ms = 1e3 * (time.perf_counter() - t0) / 3
+            plum.append({"terms": args.terms, "m2l": "split-FP16 MFMA (levels >= 4)" if mix else "FP64",
+                         "tree_depth": rep["tree_depth"], "max_leaf": rep["max_leaf"], "ms_per_step": round(ms, 3),
+                         "value": args.n / (ms * 1e-3),
+                         "sample": (pot[idx_d].cpu().numpy(), field[idx_d].cpu().numpy())})
+            print("bench: plummer terms=%d %s depth %d: %.3f ms/step" % (args.terms, "mixed" if mix else "fp64",
+                                                                       rep["tree_depth"], ms), file=sys.stderr,
+                  flush=True)
+
     accuracy = None
     other = []
     if not args.no_error:
@@ -202,6 +241,17 @@ def main():
                                                                                   head["sample"][0]))))}
         for e in extras:
             e["rel_l2_potential"], e["rel_l2_field"] = errors(e["r"]["sample"])
+        if plum:
+            t1 = time.perf_counter()
+            pdp, pdf = oracle.direct3d(xp_, qp_, tgt_index=idx, include_self=False)
+            print("bench: plummer direct sum took %.1f s" % (time.perf_counter() - t1), file=sys.stderr, flush=True)
+            for e in plum:
+                p_, f_ = e["sample"]
+                e["rel_l2_potential"] = float(np.linalg.norm(p_ - qp_[idx] * 1e4 - pdp) / np.linalg.norm(pdp))
+                e["rel_l2_field"] = float(np.linalg.norm(f_ - pdf) / np.linalg.norm(pdf))
+    for e in plum:
+        e.pop("sample")
+        e["meets_1e-6"] = bool(e.get("rel_l2_potential", 1.0) <= 1e-6)
     for e in extras:
         r = e.pop("r")
         e.update({"ms_per_step": round(r["ms"], 3), "value": r["value"],
@@ -238,6 +288,9 @@ def main():
                       "m2l": ("split-FP16 MFMA at levels >= 4, FP64 at levels 2-3" if mixed else "FP64"),
                       "parallelism": "single GPU"},
            "accuracy": accuracy, "roofline": roofline, "leaf_kernel": leaf, "other_configs": other,
+           "plummer": {"workload": "%d particles, Plummer sphere a = 0.05 truncated at r = 1, centred in the unit "
+                                   "cube (bounding box grows), q~U[-1,1], uniform depth from leaf_max 64" % args.n,
+                       "runs": plum} if plum else None,
            "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 

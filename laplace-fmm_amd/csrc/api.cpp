@@ -240,6 +240,36 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
             if (!cws) return LFMM_ERR_NOMEM;
             int lvl = 0;
             LFMM_HIP(occupancy_depth(dim, dpos, ps, n, geom, leaf_max, maxl, cws, umax, &lvl, st));
+            if (lvl > maxl) {
+                // clustered input: beyond the dense histograms, sort the keys
+                // of the deepest level once and take the largest run of each
+                // coarser key prefix
+                const int deep = dim == 3 ? kMaxDepth - 1 : 2 * kMaxDepth - 2;
+                uint64_t* dk = take<uint64_t>(A, n);
+                uint32_t* dv = take<uint32_t>(A, n);
+                void* dws = A.alloc((size_t)sort_ws_bytes(n));
+                uint32_t* fl = take<uint32_t>(A, n);
+                uint32_t* ex = take<uint32_t>(A, n);
+                uint32_t* sw = take<uint32_t>(A, scan_ws_elems(n));
+                uint64_t* uk = take<uint64_t>(A, n);
+                int32_t* ub = take<int32_t>(A, n + 1);
+                if (!dk || !dv || !dws || !fl || !ex || !sw || !uk || !ub) return LFMM_ERR_NOMEM;
+                LFMM_HIP(keys(dim, dpos, ps, n, deep, geom, dk, dv, st));
+                uint64_t* sk = dk;
+                uint32_t* so = dv;
+                LFMM_HIP(radix_sort(dk, dv, n, dim * deep, dws, st, &sk, &so));
+                lvl = deep;
+                for (int l = maxl + 1; l <= deep; ++l) {
+                    int64_t cnt = 0;
+                    LFMM_HIP(run_compact(sk, n, dim * (deep - l), fl, ex, sw, uk, ub, nullptr, &cnt, st));
+                    uint32_t mx = 0;
+                    LFMM_HIP(max_run(ub, cnt, umax, &mx, st));
+                    if ((int64_t)mx <= leaf_max) {
+                        lvl = l;
+                        break;
+                    }
+                }
+            }
             depth = lvl + 1;
         } else if (dim == 3) {  // src/cpu/multipole3d.cpp:1334-1337
             depth = (int)std::max(2.0, 1 + std::log((double)n / 100.0) / std::log(8.0));

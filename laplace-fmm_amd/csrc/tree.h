@@ -14,6 +14,8 @@ hipError_t extent(int dim, const double* pos, int64_t stride, int64_t n, double*
 hipError_t keys(int dim, const double* pos, int64_t stride, int64_t n, int bits, const double* geom, uint64_t* k,
                 uint32_t* v, hipStream_t st);
 // counts_ws: sum_{l<=max_level} 2^(dim l) uint32; maxv_ws: max_level+1 uint32.  Synchronises.
+// *level_out = the smallest level <= max_level whose fullest box holds at
+// most leaf_max particles, or max_level + 1 if there is none.
 hipError_t occupancy_depth(int dim, const double* pos, int64_t stride, int64_t n, const double* geom, int leaf_max,
                            int max_level, uint32_t* counts_ws, uint32_t* maxv_ws, int* level_out, hipStream_t st);
 int64_t scan_ws_elems(int64_t n);

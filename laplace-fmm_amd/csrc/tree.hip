@@ -581,7 +581,7 @@ hipError_t occupancy_depth(int dim, const double* pos, int64_t stride, int64_t n
     e = hipMemcpyAsync(mx.data(), maxv_ws, sizeof(uint32_t) * (max_level + 1), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return e;
-    int lvl = max_level;
+    int lvl = max_level + 1;  // none of the dense levels is fine enough
     for (int l = 0; l <= max_level; ++l)
         if ((int64_t)mx[l] <= leaf_max) { lvl = l; break; }
     *level_out = lvl;

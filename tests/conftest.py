@@ -52,6 +52,26 @@ def uniform_cube(n, seed, dim=3):
     return x, q
 
 
+def plummer(n, seed, a=0.05, rmax=1.0):
+    """Seeded Plummer sphere: scale radius a, truncated at radius rmax (radii
+    redrawn beyond it), centred at (1/2, 1/2, 1/2); charges U[-1,1].  Radii
+    from the inverse of the Plummer mass profile, r = a / sqrt(u^(-2/3) - 1),
+    with isotropic directions.  About 1% of points fall outside the unit cube;
+    they are kept (the tree's bounding box grows to hold them)."""
+    rng = np.random.default_rng(seed)
+    r = np.empty(0)
+    while len(r) < n:
+        u = rng.random(2 * n)
+        rr = a / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+        r = np.concatenate([r, rr[rr <= rmax]])
+    r = r[:n]
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    x = 0.5 + r[:, None] * d
+    q = rng.uniform(-1.0, 1.0, n)
+    return x, q
+
+
 def rel_l2(a, b):
     a = np.asarray(a).ravel()
     b = np.asarray(b).ravel()
