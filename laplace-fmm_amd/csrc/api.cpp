@@ -536,9 +536,13 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         la.far = take<double4>(A, n);
         if (!la.far) return LFMM_ERR_NOMEM;
     }
+    int32_t* p2p_ws = take<int32_t>(A, p2p_ws_elems(n, cnt[leaf_level]));
+    uint32_t* p2p_scan = take<uint32_t>(A, scan_ws_elems(cnt[leaf_level]));
+    if (!p2p_ws || !p2p_scan) return LFMM_ERR_NOMEM;
     const size_t ev_leaf = nev;
     if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
-    LFMM_HIP(launch_leaf(N, far, la, st));
+    if (far) LFMM_HIP(launch_l2p(N, la, st));
+    LFMM_HIP(launch_p2p(la, p2p_ws, p2p_scan, st));  // near field, several targets per lane
     if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
     if (memory == LFMM_MEM_HOST) {
         if (pot_out) LFMM_HIP(hipMemcpyAsync(pot_out, dpot, sizeof(double) * n_out, hipMemcpyDeviceToHost, st));

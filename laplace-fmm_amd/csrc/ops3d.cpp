@@ -6,7 +6,7 @@ namespace lfmm {
 
 template <int N> hipError_t do_p2m(const P2MArgs& a, hipStream_t st);
 template <int N> hipError_t do_rtr(int kind, const RtrTables& T, const RtrLevel& L, hipStream_t st);
-template <int N> hipError_t do_leaf(bool local, const LeafArgs& a, hipStream_t st);
+template <int N> hipError_t do_l2p(const LeafArgs& a, hipStream_t st);
 template <int N> hipError_t do_m2l16_pack(const M2L16Args& a, hipStream_t st);
 template <int N> hipError_t do_m2l16(const M2L16Args& a, hipStream_t st);
 
@@ -32,10 +32,10 @@ hipError_t launch_rtr(int N, int kind, const RtrTables& T, const RtrLevel& L, hi
     }
 }
 
-hipError_t launch_leaf(int N, bool local, const LeafArgs& a, hipStream_t st)
+hipError_t launch_l2p(int N, const LeafArgs& a, hipStream_t st)
 {
     switch (N) {
-#define X(n) case n: return do_leaf<n>(local, a, st);
+#define X(n) case n: return do_l2p<n>(a, st);
         LFMM_FOR_TERMS(X)
 #undef X
     default: return hipErrorInvalidValue;

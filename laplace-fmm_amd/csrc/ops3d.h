@@ -120,7 +120,12 @@ constexpr int m16_targets(int N)
 
 hipError_t launch_p2m(int N, const P2MArgs& a, hipStream_t st);
 hipError_t launch_rtr(int N, int kind, const RtrTables& T, const RtrLevel& L, hipStream_t st);
-hipError_t launch_leaf(int N, bool local, const LeafArgs& a, hipStream_t st);
+// L2P alone into a.far (k_l2p), for the pair kernels below
+hipError_t launch_l2p(int N, const LeafArgs& a, hipStream_t st);
+// Near field with several targets per lane (p2p.hip), starting from a.far when
+// set.  ws: p2p_ws_elems(n, nleaf) int32; scan_ws: scan_ws_elems(nleaf).
+int64_t p2p_ws_elems(int64_t n, int64_t nleaf);
+hipError_t launch_p2p(const LeafArgs& a, int32_t* ws, uint32_t* scan_ws, hipStream_t st);
 // nbr[leaf][27] = particle ranges of the 27 neighbour leaves (tree.hip)
 hipError_t launch_leaf_ranges(const uint64_t* leaf_key, const int32_t* leaf_begin, int nleaf, int level,
                               const int32_t* lut, int2* nbr, hipStream_t st);

@@ -3,8 +3,9 @@
 //
 //   k_p2m   P2M, wave per leaf           (reference Multipole, src/cpu/multipole3d.cpp:20-49)
 //   k_rtr   M2M and M2L(+L2L)            (reference M2M :115-150, M2L :152-190, L2L :192-226)
-//   k_leaf  fused P2P + L2P, lane per target particle, results scattered to input order
-//           (reference ComputeLeaves<double> :855-1294 and the un-permute :1288-1293)
+//   k_l2p   L2P, lane per target particle, far field in sorted order
+//           (reference ComputeLeaves<double> :1156-1210); the pair sums and the
+//           scatter to input order are in p2p.hip
 //
 // Coefficients are normalised per level (lfmm_internal.h), so every operator
 // table depends only on an integer box offset.
@@ -497,19 +498,7 @@ __global__ void __launch_bounds__(256) k_rtr(RTR_TABLE_PARAMS, RtrLevel L)
     }
 }
 
-// ===================================================================== P2P + L2P
-// Pair term of the reference (src/common/simple_math.h:130-139, numerics of
-// avx.h:128-146): q / max(r, 1e-4) and q (x - y) / max(r, 1e-4)^3.  1/r is
-// v_rsq_f64 (relative error ~5e-8) refined by one Newton step: relative error
-// <= 4.3e-15 (measured over 4M arguments in [1e-9, 10]), 1e-4 of the FMM's own
-// error at any order, for 3 fewer FP64 operations per pair than a second step.
-__device__ __forceinline__ double rsqrt_fp64(double r2)
-{
-    const double y = __builtin_amdgcn_rsq(r2);
-    const double hr = 0.5 * r2;
-    return y * fma(-hr * y, y, 1.5);
-}
-
+// ===================================================================== L2P
 // L2P with normalised locals: phi = (1/h) Re G(u), G = sum w_m Lt_nm K_nm S_n^m(u),
 // field = -(1/h^2) grad_u Re G, using
 //   d/dz S_n^m = (n+m) S_{n-1}^m,  (d/dx + i d/dy) S_n^m = S_{n-1}^{m+1},
@@ -619,75 +608,6 @@ __global__ void __launch_bounds__(256) k_l2p(LeafArgs a, const double2* __restri
     a.far[s] = make_double4(lp * ih, -gx * ih2, -gy * ih2, -gz * ih2);
 }
 
-template <int N, bool LOCAL>
-__global__ void __launch_bounds__(256) k_leaf(LeafArgs a, const double2* __restrict__ local, const double* __restrict__ knm)
-{
-    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (s >= a.n) return;
-    const int64_t o = a.perm[s];
-    if (o >= a.n_out) return;  // a source of a source/target solve: no output
-    const int leaf = a.leaf_of[s];
-    const uint64_t key = a.leaf_key[leaf];
-    const uint32_t ix = compact3(key >> 2), iy = compact3(key >> 1), iz = compact3(key);
-    const double4 t = a.src[s];
-    // far field first: its registers are dead before the near-field loop starts
-    double ph = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
-    if constexpr (LOCAL) {
-        const double scale = a.geom[3];
-        const double cx = box_centre(ix, a.level, scale, a.geom[0]);
-        const double cy = box_centre(iy, a.level, scale, a.geom[1]);
-        const double cz = box_centre(iz, a.level, scale, a.geom[2]);
-        const double ih = ldexp(1.0, a.level) / scale;
-        double lp, gx, gy, gz;
-        l2p_eval<N>(local + (int64_t)leaf * tri(N), knm, (t.x - cx) * ih, (t.y - cy) * ih, (t.z - cz) * ih, lp, gx, gy, gz);
-        ph = lp * ih;
-        const double ih2 = ih * ih;
-        fx = -gx * ih2;
-        fy = -gy * ih2;
-        fz = -gz * ih2;
-    } else if (a.far) {  // far field from k_l2p
-        const double4 f = a.far[s];
-        ph = f.x;
-        fx = f.y;
-        fy = f.z;
-        fz = f.w;
-    }
-    // near field over the 27 neighbour leaves (ranges from k_leaf_ranges), four
-    // sources per step with their loads issued together, then the remainder
-    auto pair = [&](const double4& p) {
-        const double dx = t.x - p.x, dy = t.y - p.y, dzz = t.z - p.z;
-        const double r2 = fmax(fma(dx, dx, fma(dy, dy, dzz * dzz)), 1.e-8);
-        const double ir = rsqrt_fp64(r2);
-        const double qi = p.w * ir;
-        const double qi3 = qi * (ir * ir);
-        ph += qi;
-        fx = fma(dx, qi3, fx);
-        fy = fma(dy, qi3, fy);
-        fz = fma(dzz, qi3, fz);
-    };
-    const int2* __restrict__ nr = a.nbr + (int64_t)leaf * 27;
-    for (int k = 0; k < 27; ++k) {
-        const int2 r = nr[k];
-        int j = r.x;
-        for (; j + 3 < r.y; j += 4) {
-            const double4 p0 = a.src[j], p1 = a.src[j + 1], p2 = a.src[j + 2], p3 = a.src[j + 3];
-            pair(p0);
-            pair(p1);
-            pair(p2);
-            pair(p3);
-        }
-        for (; j < r.y; ++j) pair(a.src[j]);
-    }
-    // the self pair (clamped at 1e-4) was included; its field term is exactly zero
-    if (a.exclude_self) ph -= t.w * rsqrt_fp64(1.e-8);
-    if (a.pot) a.pot[o] = ph;
-    if (a.field) {
-        a.field[3 * o] = fx;
-        a.field[3 * o + 1] = fy;
-        a.field[3 * o + 2] = fz;
-    }
-}
-
 // ===================================================================== launchers
 template <int N>
 hipError_t do_p2m(const P2MArgs& a, hipStream_t st)
@@ -716,20 +636,10 @@ hipError_t do_rtr(int kind, const RtrTables& T, const RtrLevel& L, hipStream_t s
 }
 
 template <int N>
-hipError_t do_leaf(bool local, const LeafArgs& a, hipStream_t st)
+hipError_t do_l2p(const LeafArgs& a, hipStream_t st)
 {
     const int nb = (int)((a.n + 255) / 256);
-    if (nb == 0) return hipSuccess;
-    if (local && a.far) {  // split: L2P, then the pair kernel at its own occupancy
-        hipLaunchKernelGGL((k_l2p<N>), dim3(nb), dim3(256), 0, st, a, a.local, a.knm);
-        hipLaunchKernelGGL((k_leaf<N, false>), dim3(nb), dim3(256), 0, st, a, a.local, a.knm);
-    } else if (local) {
-        hipLaunchKernelGGL((k_leaf<N, true>), dim3(nb), dim3(256), 0, st, a, a.local, a.knm);
-    } else {
-        LeafArgs b = a;
-        b.far = nullptr;
-        hipLaunchKernelGGL((k_leaf<N, false>), dim3(nb), dim3(256), 0, st, b, a.local, a.knm);
-    }
+    if (nb > 0) hipLaunchKernelGGL((k_l2p<N>), dim3(nb), dim3(256), 0, st, a, a.local, a.knm);
     return hipGetLastError();
 }
 
