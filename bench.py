@@ -181,7 +181,7 @@ def main():
         except (OSError, ValueError):
             pass
     p2p_pairs = counts["p2p_pairs"]
-    leaf = {"kernel": "k_leaf<N> (P2P+L2P)", "ms_per_step": round(leaf_ms, 4), "p2p_pairs": p2p_pairs,
+    leaf = {"kernel": "k_l2p<N> + k_p2p (L2P, then P2P)", "ms_per_step": round(leaf_ms, 4), "p2p_pairs": p2p_pairs,
             "gflops_p2p": round(20.0 * p2p_pairs / (leaf_ms * 1e-3) / 1e9, 1)}
 
     # secondary configurations: p = 10 (11 terms) and the all-FP64 path
