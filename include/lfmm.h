@@ -98,7 +98,7 @@ typedef struct lfmm_report {
     /* With LFMM_FLAG_TIMINGS: summed device durations (HIP events on the
      * solve's stream) of all M2L kernels (FP64 M2L+L2L, and with
      * LFMM_FLAG_MIXED the matrix-core M2L), their launch count, and of the
-     * fused P2P+L2P leaf kernel. */
+     * leaf step (L2P and the near-field pair kernels). */
     double t_m2l_kernels_ms;
     int32_t m2l_launches;
     int32_t m2l_mixed_launches;  /* matrix-core M2L launches (LFMM_FLAG_MIXED) */
@@ -221,6 +221,33 @@ int lfmm_fmm3d_vec(lfmm_ctx* ctx, const lfmm_points3d* sources, const lfmm_point
                    int memory, unsigned flags, double* force, lfmm_report* report);
 int lfmm_direct3d_vec(lfmm_ctx* ctx, const lfmm_points3d* sources, const lfmm_points3d* targets, int memory,
                       double* force);
+
+/*
+ * Many independent single-set 3D systems in one call.  The reference has one
+ * system per fmm::gpu::FastMultipole3d construction (src/cuda/multipole3d.h:16,
+ * on the legacy default stream); its drivers construct one object per
+ * particle set (src/examples/main.cpp:190, 194).  This call replaces a loop
+ * of such constructions over many systems: system s is
+ * particles offsets[s] .. offsets[s+1]-1 of the packed arrays
+ *   pos      [3 * offsets[n_sys]]  x, y, z per particle
+ *   q        [offsets[n_sys]]
+ * and its outputs go to the same rows of potential [offsets[n_sys]] and
+ * field [3 * offsets[n_sys]] (either may be NULL).  offsets (n_sys + 1
+ * entries, non-decreasing) is always a host array; the others follow
+ * `memory`.  eps, terms, tree_depth, leaf_max and flags apply to every system
+ * as in lfmm_fmm3d.  reports: NULL or n_sys entries.
+ * workers: worker contexts (own stream, workspace, host thread) that take
+ * the systems largest first; 0 = LFMM_BATCH_DEFAULT_WORKERS, at most
+ * LFMM_BATCH_MAX_WORKERS.  They are kept in ctx until lfmm_destroy.  With
+ * device memory the workers first wait for work queued on ctx's stream.  The
+ * call returns when every system is done; the first failing status is
+ * returned and the remaining systems are not started.
+ */
+#define LFMM_BATCH_DEFAULT_WORKERS 8
+#define LFMM_BATCH_MAX_WORKERS 16
+int lfmm_fmm3d_batch(lfmm_ctx* ctx, int n_sys, const int64_t* offsets, const double* pos, const double* q,
+                     double eps, int terms, int tree_depth, int leaf_max, int memory, unsigned flags, int workers,
+                     double* potential, double* field, lfmm_report* reports);
 
 /* Result kinds, in the order of the reference's fmm::InteractionType
  * (src/common/utils.h:19-24). */

@@ -32,7 +32,8 @@ STATUS = {0: "ok", 1: "invalid argument", 2: "expansion terms out of range", 3: 
 EXPORTS = ("lfmm_create", "lfmm_destroy", "lfmm_strerror", "lfmm_set_stream", "lfmm_trim", "lfmm_fmm3d",
            "lfmm_tree3d", "lfmm_fmm2d", "lfmm_tree2d", "lfmm_version", "lfmm_host_wigner_d", "lfmm_host_op_tables",
            "lfmm_host_m2l_dense", "lfmm_host_m2l16_pack", "lfmm_direct3d", "lfmm_direct2d", "lfmm_exact_filename",
-           "lfmm_write_exact", "lfmm_read_exact", "lfmm_error_stats", "lfmm_fmm3d_vec", "lfmm_direct3d_vec")
+           "lfmm_write_exact", "lfmm_read_exact", "lfmm_error_stats", "lfmm_fmm3d_vec", "lfmm_direct3d_vec",
+           "lfmm_fmm3d_batch")
 
 
 class LfmmError(RuntimeError):
@@ -138,6 +139,11 @@ def lib():
                                      ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_uint, ctypes.c_void_p, ctypes.POINTER(Report)]
         L.lfmm_fmm3d_vec.restype = ctypes.c_int
+        L.lfmm_fmm3d_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_uint, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p]
+        L.lfmm_fmm3d_batch.restype = ctypes.c_int
         L.lfmm_direct3d_vec.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points3d), ctypes.POINTER(Points3d),
                                         ctypes.c_int, ctypes.c_void_p]
         L.lfmm_direct3d_vec.restype = ctypes.c_int
@@ -340,6 +346,52 @@ def fmm3d_vec(xyz, qvec, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max
                       lambda mem, out: (float(eps), int(terms), int(tree_depth), int(leaf_max), mem, flags, out,
                                         ctypes.byref(rep)))
     return force, rep.as_dict()
+
+
+def fmm3d_batch(xyz, q, offsets, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exclude_self=False,
+                mixed=False, workers=0, ctx=None, out=None, reports=False):
+    """Many independent single-set 3D systems (lfmm_fmm3d_batch).
+
+    xyz (total,3) and q (total,) hold the systems packed one after another;
+    system s is rows offsets[s] .. offsets[s+1]-1 (offsets: n_sys + 1 ints).
+    Returns (pot (total,), field (total,3)[, list of report dicts]) in the same
+    rows.  float64 numpy arrays take the host path, CUDA torch tensors the
+    device path (outputs are then CUDA tensors, or written into out)."""
+    ctx = ctx or default_context()
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    n_sys = len(off) - 1
+    if n_sys < 0:
+        raise ValueError("offsets needs n_sys + 1 entries")
+    flags = (FLAG_EXCLUDE_SELF if exclude_self else 0) | (FLAG_MIXED if mixed else 0)
+    reps = (Report * max(n_sys, 1))() if reports else None
+    if _is_torch_cuda(xyz):
+        import torch
+        if xyz.dtype != torch.float64 or q.dtype != torch.float64 or not xyz.is_contiguous() or not q.is_contiguous():
+            raise TypeError("device inputs must be contiguous float64")
+        total = xyz.shape[0]
+        if out is None:
+            pot = torch.empty(total, dtype=torch.float64, device=xyz.device)
+            field = torch.empty((total, 3), dtype=torch.float64, device=xyz.device)
+        else:
+            pot, field = out
+        args = (xyz.data_ptr(), q.data_ptr(), MEM_DEVICE, pot.data_ptr(), field.data_ptr())
+    else:
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        total = len(q)
+        pot = np.empty(total)
+        field = np.empty((total, 3))
+        args = (xyz.ctypes.data, q.ctypes.data, MEM_HOST, pot.ctypes.data, field.ctypes.data)
+    if n_sys > 0 and (off[0] != 0 or off[-1] != total):
+        raise ValueError("offsets must run from 0 to the particle count")
+    rc = lib().lfmm_fmm3d_batch(ctx.handle, n_sys, off.ctypes.data, ctypes.c_void_p(args[0]), ctypes.c_void_p(args[1]),
+                                float(eps), int(terms), int(tree_depth), int(leaf_max), args[2], flags, int(workers),
+                                ctypes.c_void_p(args[3]), ctypes.c_void_p(args[4]),
+                                ctypes.cast(reps, ctypes.c_void_p) if reps is not None else None)
+    _check(rc, "lfmm_fmm3d_batch")
+    if reports:
+        return pot, field, [reps[s].as_dict() for s in range(n_sys)]
+    return pot, field
 
 
 def direct3d_vec(xyz, qvec, targets=None, ctx=None):

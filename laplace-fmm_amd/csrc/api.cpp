@@ -681,6 +681,7 @@ int lfmm_create(lfmm_ctx** out, int device)
 void lfmm_destroy(lfmm_ctx* c)
 {
     if (!c) return;
+    for (lfmm_ctx* w : c->workers) lfmm_destroy(w);
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     c->arena.release();

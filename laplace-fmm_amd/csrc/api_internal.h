@@ -97,6 +97,7 @@ struct lfmm_ctx {
     std::vector<hipEvent_t> events;  // timing events, created on first timed call
     std::map<int, lfmm::host::DevTables> tables;
     std::map<int, lfmm::host::DevTables2d*> tables2d;
+    std::vector<lfmm_ctx*> workers;  // lfmm_fmm3d_batch worker contexts, created on first use
     std::mutex mu;
 };
 

@@ -121,4 +121,14 @@ __device__ inline int find_box(uint64_t key, const int* __restrict__ lut, const 
     return (lo < size && keys[lo] == key) ? lo : -1;
 }
 
+// Blocks are dealt to the 8 XCDs round-robin; renumber them so that each XCD
+// walks a contiguous range of work items (a compact region of the tree in
+// Morton order), whose shared inputs then stay in that XCD's L2.
+__device__ __forceinline__ int xcd_local_block(int b, int nb)
+{
+    const int per = nb >> 3, rem = nb & 7;
+    const int x = b & 7, slot = b >> 3;
+    return (x < rem) ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
+}
+
 }  // namespace lfmm

@@ -245,16 +245,6 @@ __device__ __forceinline__ void rtr_phase_b(const double* __restrict__ tt, doubl
         const int* __restrict__ Tclass_dirs
 #define RTR_TABLE_ARGS Trot, Tephi, Ttrans, Tdir_theta, Tdir_trans, Tdir_vec, Tclass_dirs
 
-// Blocks are dealt to the 8 XCDs round-robin; renumber them so that each XCD
-// walks a contiguous range of (parent tile, child class) work items, i.e. a
-// compact region of the octree whose source expansions stay in its L2.
-__device__ __forceinline__ int xcd_local_block(int b, int nb)
-{
-    const int per = nb >> 3, rem = nb & 7;
-    const int x = b & 7, slot = b >> 3;
-    return (x < rem) ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
-}
-
 // Barrier for LDS hand-over between the phases: waits for this wave's LDS
 // traffic only, so global prefetch loads stay in flight across it.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
@@ -590,7 +580,7 @@ __device__ __forceinline__ void l2p_eval(const double2* __restrict__ Lt, const d
 template <int N>
 __global__ void __launch_bounds__(256) k_l2p(LeafArgs a, const double2* __restrict__ local, const double* __restrict__ knm)
 {
-    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t s = xcd_local_block(blockIdx.x, gridDim.x) * (int64_t)blockDim.x + threadIdx.x;
     if (s >= a.n) return;
     if (a.perm[s] >= a.n_out) return;
     const int leaf = a.leaf_of[s];

@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(256) k_p2p_multi(LeafArgs a, const int32_t* __
                                                   const uint32_t* __restrict__ slot_begin,
                                                   const uint32_t* __restrict__ cnt)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)xcd_local_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int64_t total = (int64_t)slot_begin[a.nleaf - 1] + cnt[a.nleaf - 1];
     if (i >= total) return;
     const int64_t s0 = lanes[i];
@@ -83,10 +83,31 @@ __global__ void __launch_bounds__(256) k_p2p_multi(LeafArgs a, const int32_t* __
         }
     }
     const int2* __restrict__ nr = a.nbr + (int64_t)leaf * 27;
+#ifdef LFMM_P2P_FLAT
+    // the 27 ranges walked as one list, the next range's bounds loaded ahead:
+    // a wave runs as long as its longest leaf total, not as the sum over the
+    // ranges of its longest leaf in each
+    int k = 0;
+    int2 r = nr[0], rn = nr[1];
+    int j = r.x;
+    for (;;) {
+        if (j >= r.y) {
+            do {
+                ++k;
+                r = rn;
+                rn = nr[min(k + 1, 26)];
+            } while (k < 27 && r.x >= r.y);
+            if (k >= 27) break;
+            j = r.x;
+        }
+        {
+            const double4 p = a.src[j++];
+#else
     for (int k = 0; k < 27; ++k) {
         const int2 r = nr[k];
         for (int j = r.x; j < r.y; ++j) {
             const double4 p = a.src[j];
+#endif
 #pragma unroll
             for (int u = 0; u < TPL; ++u) {
                 const double dx = t[u].x - p.x, dy = t[u].y - p.y, dz = t[u].z - p.z;
