@@ -83,7 +83,6 @@ __global__ void __launch_bounds__(256) k_p2p_multi(LeafArgs a, const int32_t* __
         }
     }
     const int2* __restrict__ nr = a.nbr + (int64_t)leaf * 27;
-#ifdef LFMM_P2P_FLAT
     // the 27 ranges walked as one list, the next range's bounds loaded ahead:
     // a wave runs as long as its longest leaf total, not as the sum over the
     // ranges of its longest leaf in each
@@ -100,26 +99,18 @@ __global__ void __launch_bounds__(256) k_p2p_multi(LeafArgs a, const int32_t* __
             if (k >= 27) break;
             j = r.x;
         }
-        {
-            const double4 p = a.src[j++];
-#else
-    for (int k = 0; k < 27; ++k) {
-        const int2 r = nr[k];
-        for (int j = r.x; j < r.y; ++j) {
-            const double4 p = a.src[j];
-#endif
+        const double4 p = a.src[j++];
 #pragma unroll
-            for (int u = 0; u < TPL; ++u) {
-                const double dx = t[u].x - p.x, dy = t[u].y - p.y, dz = t[u].z - p.z;
-                const double r2 = fmax(fma(dx, dx, fma(dy, dy, dz * dz)), 1.e-8);
-                const double ir = rsqrt_nf(r2);
-                const double qi = p.w * ir;
-                const double qi3 = qi * (ir * ir);
-                ph[u] += qi;
-                fx[u] = fma(dx, qi3, fx[u]);
-                fy[u] = fma(dy, qi3, fy[u]);
-                fz[u] = fma(dz, qi3, fz[u]);
-            }
+        for (int u = 0; u < TPL; ++u) {
+            const double dx = t[u].x - p.x, dy = t[u].y - p.y, dz = t[u].z - p.z;
+            const double r2 = fmax(fma(dx, dx, fma(dy, dy, dz * dz)), 1.e-8);
+            const double ir = rsqrt_nf(r2);
+            const double qi = p.w * ir;
+            const double qi3 = qi * (ir * ir);
+            ph[u] += qi;
+            fx[u] = fma(dx, qi3, fx[u]);
+            fy[u] = fma(dy, qi3, fy[u]);
+            fz[u] = fma(dz, qi3, fz[u]);
         }
     }
     // the self pair (clamped at 1e-4) was included; its field term is exactly zero
