@@ -235,6 +235,29 @@ def main():
                                                                        rep["tree_depth"], ms), file=sys.stderr,
                   flush=True)
 
+    # 2D, 1,048,576 uniform points, p = 20 (21 terms), leaf_max 64: FP64
+    # against FP32 per-direction M2L sums at levels >= 4
+    runs2d = []
+    if not args.no_extra:
+        r2 = np.random.default_rng(args.seed + 31)
+        x2, q2 = r2.random((1048576, 2)), r2.uniform(-1.0, 1.0, 1048576)
+        x2d, q2d = torch.from_numpy(x2).to(dev), torch.from_numpy(q2).to(dev)
+        for mix in (False, True):
+            def step2():
+                return lfmm.fmm2d(x2d, q2d, terms=21, leaf_max=64, exclude_self=True, ctx=ctx, mixed=mix)
+            p2_, f2_, rep2 = step2()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                step2()
+            torch.cuda.synchronize()
+            ms2 = 1e3 * (time.perf_counter() - t0) / 5
+            runs2d.append({"m2l": "FP32 sums at levels >= 4, FP64 accumulation" if mix else "FP64",
+                           "tree_depth": rep2["tree_depth"], "ms_per_step": round(ms2, 3),
+                           "value": 1048576 / (ms2 * 1e-3), "sample": (p2_.cpu().numpy(), f2_.cpu().numpy())})
+            print("bench: 2d 1M p=20 %s %.3f ms/step" % ("fp32-m2l" if mix else "fp64", ms2), file=sys.stderr,
+                  flush=True)
+
     # many independent systems in one call (p = 8, leaf_max 32), against a
     # loop of single solves over the same device-resident systems
     batch = None
@@ -302,6 +325,15 @@ def main():
                 p_, f_ = e["sample"]
                 e["rel_l2_potential"] = float(np.linalg.norm(p_ - qp_[idx] * 1e4 - pdp) / np.linalg.norm(pdp))
                 e["rel_l2_field"] = float(np.linalg.norm(f_ - pdf) / np.linalg.norm(pdf))
+    if runs2d and not args.no_error:
+        i2 = np.random.default_rng(args.seed + 32).choice(1048576, 1024, replace=False)
+        dp2, df2 = oracle.direct2d(x2, q2, tgt_index=i2, include_self=False)
+        for e in runs2d:
+            p_, f_ = e["sample"]
+            e["rel_l2_potential"] = float(np.linalg.norm(p_[i2] - dp2) / np.linalg.norm(dp2))
+            e["rel_l2_force"] = float(np.linalg.norm(f_[i2] - df2) / np.linalg.norm(df2))
+    for e in runs2d:
+        e.pop("sample")
     if batch is not None and not args.no_error:
         pbh, fbh = pb.cpu().numpy(), fb.cpu().numpy()
         rs_ = np.random.default_rng(args.seed + 22)
@@ -361,6 +393,8 @@ def main():
            "plummer": {"workload": "%d particles, Plummer sphere a = 0.05 truncated at r = 1, centred in the unit "
                                    "cube (bounding box grows), q~U[-1,1], uniform depth from leaf_max 64" % args.n,
                        "runs": plum, "mixed_speedup_at_1e-6": plum_speedup} if plum else None,
+           "config2d": {"workload": "2D Laplace FMM, 1048576 points uniform in [0,1)^2, q~U[-1,1], p=20 (21 terms), "
+                                    "leaf_max 64, self term excluded", "runs": runs2d} if runs2d else None,
            "batch": batch,
            "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)

@@ -167,7 +167,9 @@ int lfmm_tree3d(lfmm_ctx* ctx, const lfmm_points3d* particles, int tree_depth, i
  *   potential: n_out doubles, sum q log|z - z_j| (self pair clamped at 1e-4);
  *   field: n_out*2 doubles, (re, im) of the complex force sum q / (z - z_j),
  *     the reference's std::complex<double> forces;
- *   LFMM_FLAG_MIXED is not supported (LFMM_ERR_UNSUPPORTED).
+ *   LFMM_FLAG_MIXED: the M2L of levels >= 4 forms each direction's sum in
+ *     FP32 (FP32 operator, sources rounded on load) and adds it to the FP64
+ *     local; levels 2-3, L2L, P2M, M2M, L2P and P2P stay FP64.
  */
 int lfmm_fmm2d(lfmm_ctx* ctx, const lfmm_points2d* sources, const lfmm_points2d* targets,
                double eps, int terms, int tree_depth, int leaf_max,

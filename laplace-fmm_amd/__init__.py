@@ -262,12 +262,13 @@ def fmm3d(xyz, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exc
 
 
 def fmm2d(xy, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exclude_self=False,
-          timings=False, count_ops=False, ctx=None, out=None, targets=None):
+          timings=False, count_ops=False, ctx=None, out=None, targets=None, mixed=False):
     """2D FMM (complex series): potential (m,) = sum q log|z - z_j| and the
     complex force (m,2) = (re, im) of sum q / (z - z_j) at every target.
-    Arguments as fmm3d with 2 coordinates per point."""
+    Arguments as fmm3d with 2 coordinates per point; mixed=True forms the
+    M2L sums of levels >= 4 in FP32 (LFMM_FLAG_MIXED)."""
     return _solve(2, xy, q, eps, terms, tree_depth, leaf_max, exclude_self, timings, count_ops, ctx, out, targets,
-                  False)
+                  mixed)
 
 
 # result kinds (the reference's fmm::InteractionType order)

@@ -29,10 +29,11 @@ static int get_tables2d(lfmm_ctx* ctx, int N, const DevTables2d** out)
     OpTables2d T;
     build_op_tables_2d(N, T);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const std::vector<float> m2l32(T.m2l.begin(), T.m2l.end());  // round to nearest
     const size_t b1 = T.m2l.size() * 8, b2 = T.m2m.size() * 8, b3 = T.l2l.size() * 8, b4 = T.dir_vec.size() * 4,
-                 b5 = T.class_dirs.size() * 4;
+                 b5 = T.class_dirs.size() * 4, b6 = m2l32.size() * 4;
     char* blk = nullptr;
-    if (hipMalloc(&blk, al(b1) + al(b2) + al(b3) + al(b4) + al(b5)) != hipSuccess) return LFMM_ERR_NOMEM;
+    if (hipMalloc(&blk, al(b1) + al(b2) + al(b3) + al(b4) + al(b5) + al(b6)) != hipSuccess) return LFMM_ERR_NOMEM;
     size_t o = 0;
     bool ok = true;
     auto put = [&](const void* src, size_t bytes) -> void* {
@@ -53,6 +54,7 @@ static int get_tables2d(lfmm_ctx* ctx, int N, const DevTables2d** out)
     D->t.l2l = (const double2*)put(T.l2l.data(), b3);
     D->t.dir_vec = (const int*)put(T.dir_vec.data(), b4);
     D->t.class_dirs = (const int*)put(T.class_dirs.data(), b5);
+    D->t.m2l32 = (const float2*)put(m2l32.data(), b6);
     if (!ok || hipDeviceSynchronize() != hipSuccess) {
         (void)hipFree(blk);
         delete D;
@@ -79,12 +81,17 @@ static int terms_2d(double eps, int terms)
     return std::min(LFMM_MAX_TERMS_2D, terms);
 }
 
+// With LFMM_FLAG_MIXED the M2L of levels >= 4 forms each direction's sum in
+// FP32 (the two coarsest M2L levels, 2 and 3, stay FP64, as in 3D).
+constexpr int kMixed2dFirstLevel = 4;
+
 static int fmm2d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, double eps, int terms_in,
                        int depth_in, int leaf_max, int memory, unsigned flags, double* pot_out, double* field_out,
                        lfmm_report* rep)
 {
     const auto T0 = std::chrono::steady_clock::now();
     const bool timed = (flags & LFMM_FLAG_TIMINGS) != 0;
+    const bool mixed = (flags & LFMM_FLAG_MIXED) != 0;
     const int64_t n_out = Tg ? Tg->n : S->n;
     const int N = terms_2d(eps, terms_in);
     if (N < 1) return LFMM_ERR_TERMS;
@@ -129,7 +136,7 @@ static int fmm2d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3
         t_up = ms_since(T0) - t_tree;
     }
     size_t nev = 0;
-    int launches = 0;
+    int launches = 0, mixed_launches = 0;
     if (far) {
         for (int l = 2; l < depth; ++l) {
             M2L2dArgs ma{};
@@ -145,6 +152,8 @@ static int fmm2d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3
             ma.src = outer[l];
             ma.parent_local = (l > 2) ? inner[l - 1] : nullptr;
             ma.dst = inner[l];
+            ma.f32 = mixed && l >= kMixed2dFirstLevel;
+            mixed_launches += ma.f32 ? 1 : 0;
             if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
             LFMM_HIP(launch_m2l2d(ma, DT->t, st));
             if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
@@ -199,6 +208,7 @@ static int fmm2d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3
         rep->p2p_pairs = -1;
         rep->m2l_ops_mixed = -1;
         rep->m2l_launches = launches;
+        rep->m2l_mixed_launches = mixed_launches;
         if (timed) {
             double tm = 0.0;
             for (size_t i = 0; i + 1 < ev_leaf; i += 2) {
@@ -241,7 +251,6 @@ int lfmm_fmm2d(lfmm_ctx* c, const lfmm_points2d* sources, const lfmm_points2d* t
     if (terms != LFMM_AUTO && terms < 1) return LFMM_ERR_TERMS;
     if (tree_depth != LFMM_AUTO && (tree_depth < 1 || tree_depth > 2 * kMaxDepth - 1)) return LFMM_ERR_DEPTH;
     if (!(eps > 0.0) && terms == LFMM_AUTO) return LFMM_ERR_INVALID;
-    if (flags & LFMM_FLAG_MIXED) return LFMM_ERR_UNSUPPORTED;
     if (targets && !valid_points(targets, false, 2)) return LFMM_ERR_INVALID;
     const lfmm_points2d* tg = (targets && targets->n > 0) ? targets : nullptr;
     if (tg && sources->n + tg->n >= ((int64_t)1 << 31)) return LFMM_ERR_INVALID;

@@ -8,6 +8,7 @@ namespace lfmm {
 // Device copies of OpTables2d (complex values as double2).
 struct Tables2d {
     const double2* m2l;        // [40][N][N]
+    const float2* m2l32;       // the same rounded to FP32 (mixed M2L)
     const double2* m2m;        // [4][N][N]
     const double2* l2l;        // [4][N][N]
     const int* dir_vec;        // [40][2]
@@ -47,6 +48,7 @@ struct M2L2dArgs {
     const double2* src;        // multipoles at `level`
     const double2* parent_local;  // locals at level - 1 (null: no L2L)
     double2* dst;              // locals at `level` (written)
+    bool f32;                  // M2L products and per-direction sums in FP32
 };
 
 struct Leaf2dArgs {

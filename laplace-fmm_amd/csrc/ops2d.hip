@@ -2,7 +2,8 @@
 //
 //   k_p2m2d    P2M, thread per leaf                    (reference Multipole, src/cpu/multipole.cpp:22-36)
 //   k_m2m2d    M2M, thread per (parent, coefficient)    (M2M :38-53, Upward :155-177)
-//   k_m2l2d    L2L + M2L, thread per (target, coefficient) (M2L :66-84, L2L :86-97, Downward :179-231)
+//   k_m2l2d    L2L + M2L, thread per (target, coefficient) (M2L :66-84, L2L :86-97, Downward :179-231);
+//              FP64, or FP32 per-direction sums added in FP64
 //   k_leaf2d   L2P + P2P over the 3x3 neighbourhood, thread per particle,
 //              results scattered to input order       (ComputePotentials / ComputeForces :233-449)
 //
@@ -73,6 +74,10 @@ __global__ void __launch_bounds__(256) k_m2m2d(M2M2dArgs a, const double2* __res
     a.dst[(int64_t)P * N + j] = acc;
 }
 
+// F32: each direction's sum over the source coefficients is formed in FP32
+// (FP32 operator, sources rounded on load) and added to the FP64 accumulator;
+// the level term and the L2L stay FP64.
+template <bool F32>
 __global__ void __launch_bounds__(256) k_m2l2d(M2L2dArgs a, Tables2d T)
 {
     const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -100,13 +105,25 @@ __global__ void __launch_bounds__(256) k_m2l2d(M2L2dArgs a, Tables2d T)
         if (sx < 0 || sy < 0 || sx >= side || sy >= side) continue;
         const int s = find_box(morton2(sx, sy), a.lut, a.keys, a.size);
         if (s < 0) continue;
-        const double2* m = T.m2l + ((int64_t)d * N + l) * N;
         const double2* x = a.src + (int64_t)s * N;
         if (l == 0) {  // level term at_0 log h of the normalised M2L
             acc.x = fma(x[0].x, a.log_h, acc.x);
             acc.y = fma(x[0].y, a.log_h, acc.y);
         }
-        for (int j = 0; j < N; ++j) acc = cmadd(acc, m[j], x[j]);
+        if constexpr (F32) {
+            const float2* m = T.m2l32 + ((int64_t)d * N + l) * N;
+            float ax = 0.f, ay = 0.f;
+            for (int j = 0; j < N; ++j) {
+                const float xr = (float)x[j].x, xi = (float)x[j].y;
+                ax = fmaf(m[j].x, xr, fmaf(-m[j].y, xi, ax));
+                ay = fmaf(m[j].x, xi, fmaf(m[j].y, xr, ay));
+            }
+            acc.x += (double)ax;
+            acc.y += (double)ay;
+        } else {
+            const double2* m = T.m2l + ((int64_t)d * N + l) * N;
+            for (int j = 0; j < N; ++j) acc = cmadd(acc, m[j], x[j]);
+        }
     }
     a.dst[(int64_t)t * N + l] = acc;
 }
@@ -184,7 +201,9 @@ hipError_t launch_m2m2d(const M2M2dArgs& a, const Tables2d& T, hipStream_t st)
 hipError_t launch_m2l2d(const M2L2dArgs& a, const Tables2d& T, hipStream_t st)
 {
     const int64_t n = (int64_t)a.num_parents * 4 * a.N;
-    if (n > 0) hipLaunchKernelGGL(k_m2l2d, dim3(nblk(n, 256)), dim3(256), 0, st, a, T);
+    if (n <= 0) return hipSuccess;
+    if (a.f32) hipLaunchKernelGGL(k_m2l2d<true>, dim3(nblk(n, 256)), dim3(256), 0, st, a, T);
+    else hipLaunchKernelGGL(k_m2l2d<false>, dim3(nblk(n, 256)), dim3(256), 0, st, a, T);
     return hipGetLastError();
 }
 

@@ -110,10 +110,6 @@ def test_fmm2d_source_target_exclude_self_and_layouts(lfmm, orc, request):
     pd, fd, _ = lfmm.fmm2d(torch.from_numpy(src).cuda(), torch.from_numpy(q).cuda(), terms=16, tree_depth=5)
     torch.cuda.synchronize()
     assert np.array_equal(pd.cpu().numpy(), p1) and np.array_equal(fd.cpu().numpy(), f1)
-    # the mixed flag is 3D only
-    with pytest.raises(lfmm.LfmmError) as e:
-        lfmm._solve(2, src, q, 1e-8, 8, 4, 0, False, False, False, None, None, None, True)
-    assert e.value.code == 8
 
 
 def test_fmm2d_auto_parameters(lfmm, request):
@@ -156,3 +152,23 @@ int main() {
     a = np.loadtxt(out.stdout.splitlines())
     pot, field, _ = lfmm.fmm2d(a[:, :2], a[:, 2], terms=14, tree_depth=5)
     assert np.array_equal(pot, a[:, 3]) and np.array_equal(field, a[:, 4:6])
+
+
+@pytest.mark.parametrize("n,N,depth", [(200000, 21, 8), (60000, 30, 6)])
+def test_fmm2d_fp32_m2l(lfmm, orc, request, n, N, depth):
+    """LFMM_FLAG_MIXED in 2D: FP32 per-direction M2L sums at levels >= 4.
+    Only those launches change (count pinned), the result stays within 1e-6
+    of the FP64 solve on the same tree, and its error against the FP64
+    direct sum is the FP64 solve's plus at most 1e-6."""
+    case_name(request)
+    x, q = square(n, 53)
+    p64, f64, r64 = lfmm.fmm2d(x, q, terms=N, tree_depth=depth, exclude_self=True, timings=True)
+    p32, f32, r32 = lfmm.fmm2d(x, q, terms=N, tree_depth=depth, exclude_self=True, timings=True, mixed=True)
+    assert r32["m2l_mixed_launches"] == depth - 4 and r64["m2l_mixed_launches"] == 0
+    ep, ef = rel_l2(p32, p64), rel_l2(f32, f64)
+    print("2D n=%d N=%d: fp32-M2L vs fp64 potential %.3g force %.3g" % (n, N, ep, ef), flush=True)
+    assert 0 < ep <= 1e-6 and ef <= 1e-6
+    idx = np.random.default_rng(4).choice(n, 300, replace=False)
+    dp, df = orc.direct2d(x, q, tgt_index=idx, include_self=False)
+    assert rel_l2(p32[idx], dp) <= rel_l2(p64[idx], dp) + 1e-6
+    assert rel_l2(f32[idx], df) <= rel_l2(f64[idx], df) + 1e-6
