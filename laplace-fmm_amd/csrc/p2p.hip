@@ -32,13 +32,20 @@ __device__ __forceinline__ double rsqrt_nf(double r2)
     return y * fma(-hr * y, y, 1.5);
 }
 
+// cnt[leaf]: lanes of the leaf; total[leaf]: sources in its 27 neighbour ranges
 template <int TPL>
-__global__ void __launch_bounds__(256) k_pair_counts(const int32_t* __restrict__ leaf_begin, int nleaf,
-                                                    uint32_t* __restrict__ cnt)
+__global__ void __launch_bounds__(256) k_pair_counts(const int32_t* __restrict__ leaf_begin, const int2* __restrict__ nbr,
+                                                    int nleaf, uint32_t* __restrict__ cnt, int32_t* __restrict__ total)
 {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nleaf) return;
     cnt[l] = (uint32_t)((leaf_begin[l + 1] - leaf_begin[l] + TPL - 1) / TPL);
+    int t = 0;
+    for (int k = 0; k < 27; ++k) {
+        const int2 r = nbr[(int64_t)l * 27 + k];
+        t += max(r.y - r.x, 0);
+    }
+    total[l] = t;
 }
 
 template <int TPL>
@@ -57,7 +64,7 @@ __global__ void __launch_bounds__(256) k_pair_lanes(const int32_t* __restrict__ 
 template <int TPL>
 __global__ void __launch_bounds__(256) k_p2p_multi(LeafArgs a, const int32_t* __restrict__ lanes,
                                                   const uint32_t* __restrict__ slot_begin,
-                                                  const uint32_t* __restrict__ cnt)
+                                                  const uint32_t* __restrict__ cnt, const int32_t* __restrict__ near_total)
 {
     const int64_t i = (int64_t)xcd_local_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int64_t total = (int64_t)slot_begin[a.nleaf - 1] + cnt[a.nleaf - 1];
@@ -83,23 +90,23 @@ __global__ void __launch_bounds__(256) k_p2p_multi(LeafArgs a, const int32_t* __
         }
     }
     const int2* __restrict__ nr = a.nbr + (int64_t)leaf * 27;
-    // the 27 ranges walked as one list, the next range's bounds loaded ahead:
-    // a wave runs as long as its longest leaf total, not as the sum over the
-    // ranges of its longest leaf in each
+    // the 27 ranges walked as one list of total[leaf] sources, the next
+    // range's bounds loaded ahead: a wave runs as long as its longest leaf
+    // total, not as the sum over the ranges of its longest leaf in each.  The
+    // loop runs on a plain count (the range advance never leaves it), which
+    // keeps the accumulators in place across iterations.
     int k = 0;
     int2 r = nr[0], rn = nr[1];
     int j = r.x;
-    for (;;) {
-        if (j >= r.y) {
-            do {
-                ++k;
-                r = rn;
-                rn = nr[min(k + 1, 26)];
-            } while (k < 27 && r.x >= r.y);
-            if (k >= 27) break;
+    const int nsrc = near_total[leaf];
+    for (int it = 0; it < nsrc; ++it) {
+        while (j >= r.y && k < 26) {  // total counts these ranges: k stays <= 26
+            ++k;
+            r = rn;
+            rn = nr[min(k + 1, 26)];
             j = r.x;
         }
-        const double4 p = a.src[j++];
+        const double4 p = a.src[min(j++, (int)a.n - 1)];
 #pragma unroll
         for (int u = 0; u < TPL; ++u) {
             const double dx = t[u].x - p.x, dy = t[u].y - p.y, dz = t[u].z - p.z;
@@ -141,22 +148,24 @@ constexpr int kTPL = LFMM_P2P_TPL;  // targets per lane
 // upper bound on the lane count: sum over leaves of ceil(count / kTPL)
 inline int64_t lane_bound(int64_t n, int64_t nleaf) { return (n + (kTPL - 1) * nleaf) / kTPL + 1; }
 
-int64_t p2p_ws_elems(int64_t n, int64_t nleaf) { return lane_bound(n, nleaf) + 2 * nleaf; }
+int64_t p2p_ws_elems(int64_t n, int64_t nleaf) { return lane_bound(n, nleaf) + 3 * nleaf; }
 
 hipError_t launch_p2p(const LeafArgs& a, int32_t* ws, uint32_t* scan_ws, hipStream_t st)
 {
     if (a.n <= 0 || a.nleaf <= 0) return hipSuccess;
-    // ws: cnt [nleaf], slot_begin [nleaf], lanes [lane_bound]
+    // ws: cnt [nleaf], slot_begin [nleaf], total [nleaf], lanes [lane_bound]
     uint32_t* cnt = (uint32_t*)ws;
     uint32_t* slot = cnt + a.nleaf;
-    int32_t* lanes = (int32_t*)(slot + a.nleaf);
-    hipLaunchKernelGGL((k_pair_counts<kTPL>), dim3(nblk(a.nleaf, 256)), dim3(256), 0, st, a.leaf_begin, a.nleaf, cnt);
+    int32_t* total = (int32_t*)(slot + a.nleaf);
+    int32_t* lanes = total + a.nleaf;
+    hipLaunchKernelGGL((k_pair_counts<kTPL>), dim3(nblk(a.nleaf, 256)), dim3(256), 0, st, a.leaf_begin, a.nbr,
+                       a.nleaf, cnt, total);
     hipError_t e = exclusive_scan(cnt, slot, a.nleaf, scan_ws, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_pair_lanes<kTPL>), dim3(nblk(a.n, 256)), dim3(256), 0, st, a.leaf_of, a.leaf_begin, slot,
                        a.n, lanes);
     const int64_t bound = lane_bound(a.n, a.nleaf);
-    hipLaunchKernelGGL((k_p2p_multi<kTPL>), dim3(nblk(bound, 256)), dim3(256), 0, st, a, lanes, slot, cnt);
+    hipLaunchKernelGGL((k_p2p_multi<kTPL>), dim3(nblk(bound, 256)), dim3(256), 0, st, a, lanes, slot, cnt, total);
     return hipGetLastError();
 }
 
