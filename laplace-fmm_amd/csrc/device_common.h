@@ -98,12 +98,18 @@ __device__ inline double2 cmulc(double2 a, double2 b)  // a * conj(b)
     return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 
-// *out = max(*out, m) over the wave, one atomic per wave (m >= 0; non-negative
+// *out = max(*out, m) over the wave, at most one atomic per wave (m >= 0; non-negative
 // floats order like their bit patterns).  Call from every lane of the wave.
 __device__ inline void wave_absmax(double m, uint32_t* out)
 {
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint((float)m));
+    if ((threadIdx.x & 63) == 0) {
+        // the word only grows: skip the atomic when it already holds at least
+        // this value (a stale read only lets a needless atomic through), so a
+        // level's waves do not all serialise on one address
+        const uint32_t b = __float_as_uint((float)m);
+        if (b > __atomic_load_n(out, __ATOMIC_RELAXED)) atomicMax(out, b);
+    }
 }
 
 // Find box index by key at a level: dense table when present, otherwise a
