@@ -112,6 +112,15 @@ __device__ inline void wave_absmax(double m, uint32_t* out)
     }
 }
 
+// *out = max(*out, m) over the wave, at most one atomic per wave; skipped
+// when the word already holds at least m (it only grows).  Call from every
+// lane of the wave.
+__device__ inline void wave_max_u32(uint32_t m, uint32_t* out)
+{
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
+    if ((threadIdx.x & 63) == 0 && m > __atomic_load_n(out, __ATOMIC_RELAXED)) atomicMax(out, m);
+}
+
 // Find box index by key at a level: dense table when present, otherwise a
 // binary search over the sorted unique keys.  Returns -1 when absent.
 __device__ inline int find_box(uint64_t key, const int* __restrict__ lut, const uint64_t* __restrict__ keys, int size)

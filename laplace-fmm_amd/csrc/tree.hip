@@ -134,8 +134,7 @@ __global__ void __launch_bounds__(256) k_occupancy_up(const uint32_t* __restrict
         coarse[i] = s;
         m = s;
     }
-    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(maxv, m);
+    wave_max_u32(m, maxv);
 }
 
 __global__ void __launch_bounds__(256) k_max_u32(const uint32_t* __restrict__ v, int64_t n, uint32_t* __restrict__ maxv)
@@ -143,8 +142,7 @@ __global__ void __launch_bounds__(256) k_max_u32(const uint32_t* __restrict__ v,
     uint32_t m = 0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         m = max(m, v[i]);
-    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(maxv, m);
+    wave_max_u32(m, maxv);
 }
 
 // ------------------------------------------------------------------ scan
@@ -434,8 +432,7 @@ __global__ void __launch_bounds__(256) k_max_run(const int32_t* __restrict__ beg
     uint32_t m = 0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
         m = max(m, (uint32_t)(begin[i + 1] - begin[i]));
-    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(maxv, m);
+    wave_max_u32(m, maxv);
 }
 
 // ------------------------------------------------------------------ counts
