@@ -69,8 +69,7 @@ int lfmm_fmm3d_batch(lfmm_ctx* c, int n_sys, const int64_t* offsets, const doubl
             status.store(LFMM_ERR_HIP);
             return;
         }
-        for (;;) {
-            if (status.load() != LFMM_OK) return;
+        while (status.load() == LFMM_OK) {
             const int i = next.fetch_add(1);
             if (i >= n_sys) break;
             const int s = order[i];
@@ -83,14 +82,23 @@ int lfmm_fmm3d_batch(lfmm_ctx* c, int n_sys, const int64_t* offsets, const doubl
             if (st != LFMM_OK) {
                 int ok = LFMM_OK;
                 status.compare_exchange_strong(ok, st);
-                return;
             }
         }
-        if (hipStreamSynchronize(w->stream) != hipSuccess) status.store(LFMM_ERR_HIP);
+        // also after a failure: nothing of this worker may still run when the call returns
+        if (hipStreamSynchronize(w->stream) != hipSuccess) {
+            int ok = LFMM_OK;
+            status.compare_exchange_strong(ok, LFMM_ERR_HIP);
+        }
     };
     std::vector<std::thread> pool;
-    pool.reserve(nw);
-    for (int k = 0; k < nw; ++k) pool.emplace_back(run, c->workers[k]);
+    try {
+        pool.reserve(nw);
+        for (int k = 0; k < nw; ++k) pool.emplace_back(run, c->workers[k]);
+    } catch (...) {
+        // fewer threads than asked: those started take every system; with
+        // none started, this thread runs them
+        if (pool.empty()) run(c->workers[0]);
+    }
     for (auto& t : pool) t.join();
     if (ready) {
         for (int k = 0; k < nw; ++k) (void)hipStreamSynchronize(c->workers[k]->stream);

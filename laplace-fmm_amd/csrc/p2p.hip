@@ -5,11 +5,11 @@
 // phi_i += q_j / max(r, 1e-4), E_i += q_j (x_i - x_j) / max(r, 1e-4)^3 over
 // the 27 neighbour leaves, self pair included.
 //
-// One lane per target loads a 32-byte source record for every pair it adds,
-// so the loop is bound by vector-memory instructions, not by FP64 arithmetic.
+// One lane per target loads a 32-byte source record for every pair it adds.
 // Here each lane owns kTPL (default 2) consecutive targets of one leaf (they
 // share the neighbour list) and every source record it loads feeds kTPL pair
-// terms; 3 and 4 measured no faster than 2 at 4M uniform points, depth 7.
+// terms; 3 and 4 measured no faster than 2 at 4M uniform points, depth 7,
+// where the loop is then bound by its FP64 instructions.
 // Past a leaf's end a lane repeats its last target (computed, not written).
 // Lanes are listed per leaf: lane slot_begin[leaf] + k owns targets
 // leaf_begin[leaf] + kTPL k ... + kTPL k + kTPL - 1.
