@@ -161,6 +161,7 @@ class Context:
 
     def __init__(self, device=0):
         self._h = ctypes.c_void_p()
+        self.device = int(device)
         _check(lib().lfmm_create(ctypes.byref(self._h), int(device)), "lfmm_create")
 
     @property
@@ -196,9 +197,83 @@ def _is_torch_cuda(x):
     return hasattr(x, "is_cuda") and x.is_cuda
 
 
+def _bind_stream(ctx, t):
+    """Runs the context's next calls on torch's current stream of t's device,
+    so they follow the torch work that produced the inputs and precede the
+    torch work that consumes the outputs (a device solve returns before its
+    kernels finish)."""
+    import torch
+    if t.device.index is not None and t.device.index != ctx.device:
+        raise ValueError("tensors are on %s but the context is bound to GPU %d" % (t.device, ctx.device))
+    ctx.set_stream(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def _dev_points(name, t, dim, device):
+    """Checks a CUDA float64 (n, dim) point tensor, contiguous per point."""
+    import torch
+    if not _is_torch_cuda(t):
+        raise TypeError("%s must be a CUDA tensor like the other device inputs" % name)
+    if t.dtype != torch.float64:
+        raise TypeError("%s must be float64" % name)
+    if t.dim() != 2 or t.shape[1] != dim:
+        raise ValueError("%s must have shape (n, %d), got %s" % (name, dim, tuple(t.shape)))
+    if t.shape[0] > 0 and t.stride(1) != 1:
+        raise ValueError("%s must be contiguous per point" % name)
+    if t.device != device:
+        raise ValueError("%s is on %s, the positions on %s" % (name, t.device, device))
+    return t.shape[0]
+
+
+def _dev_vector(name, t, n, device, width=None):
+    """Checks a CUDA float64 tensor of n rows (1-D, or (n, width) contiguous per row)."""
+    import torch
+    if not _is_torch_cuda(t):
+        raise TypeError("%s must be a CUDA tensor like the other device inputs" % name)
+    if t.dtype != torch.float64:
+        raise TypeError("%s must be float64" % name)
+    if t.device != device:
+        raise ValueError("%s is on %s, the positions on %s" % (name, t.device, device))
+    if width is None:
+        if t.dim() != 1 or t.shape[0] != n:
+            raise ValueError("%s must have shape (%d,), got %s" % (name, n, tuple(t.shape)))
+        if n > 0 and t.stride(0) < 1:
+            raise ValueError("%s must have a positive stride" % name)
+    else:
+        if t.dim() != 2 or t.shape[0] != n or t.shape[1] != width:
+            raise ValueError("%s must have shape (%d, %d), got %s" % (name, n, width, tuple(t.shape)))
+        if n > 0 and t.stride(1) != 1:
+            raise ValueError("%s must be contiguous per row" % name)
+
+
+def _dev_out(name, t, n, device, width=None):
+    """Checks an output tensor: CUDA float64, n rows, densely packed."""
+    _dev_vector(name, t, n, device, width)
+    if n > 0 and not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % name)
+
+
+def _host_points(name, a, dim):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.ndim != 2 or a.shape[1] != dim:
+        if a.ndim == 1 and a.size % dim == 0:
+            a = a.reshape(-1, dim)
+        else:
+            raise ValueError("%s must have shape (n, %d), got %s" % (name, dim, a.shape))
+    return a
+
+
+def _host_charges(name, a, n, width=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    want = (n,) if width is None else (n, width)
+    if a.shape != want:
+        if width is not None and a.ndim == 1 and a.size == n * width:
+            return a.reshape(n, width)
+        raise ValueError("%s must have shape %s to match the positions, got %s" % (name, want, a.shape))
+    return a
+
+
 def _solve(dim, xyz, q, eps, terms, tree_depth, leaf_max, exclude_self, timings, count_ops, ctx, out, targets,
            mixed):
-    ctx = ctx or default_context()
     fn = lib().lfmm_fmm3d if dim == 3 else lib().lfmm_fmm2d
     name = "lfmm_fmm3d" if dim == 3 else "lfmm_fmm2d"
     flags = ((FLAG_EXCLUDE_SELF if exclude_self else 0) | (FLAG_TIMINGS if timings else 0) |
@@ -206,40 +281,44 @@ def _solve(dim, xyz, q, eps, terms, tree_depth, leaf_max, exclude_self, timings,
     rep = Report()
     if _is_torch_cuda(xyz):
         import torch
-        if xyz.dtype != torch.float64 or q.dtype != torch.float64:
-            raise TypeError("device inputs must be float64")
-        if xyz.stride(-1) != 1 or q.stride(0) < 1:
-            raise ValueError("positions must be contiguous in the last dimension")
-        n = xyz.shape[0]
+        n = _dev_points("positions", xyz, dim, xyz.device)
+        _dev_vector("charges", q, n, xyz.device)
         pts = Points3d(n, xyz.data_ptr(), xyz.stride(0), q.data_ptr(), q.stride(0))
         tg = None
         m = n
         if targets is not None:
-            if targets.dtype != torch.float64 or targets.stride(-1) != 1:
-                raise TypeError("device targets must be float64, contiguous per point")
-            m = targets.shape[0]
+            m = _dev_points("targets", targets, dim, xyz.device)
             tg = ctypes.byref(Points3d(m, targets.data_ptr(), targets.stride(0), None, 1))
         if out is None:
             pot = torch.empty(m, dtype=torch.float64, device=xyz.device)
             field = torch.empty((m, dim), dtype=torch.float64, device=xyz.device)
         else:
             pot, field = out
+            _dev_out("out potential", pot, m, xyz.device)
+            _dev_out("out field", field, m, xyz.device, dim)
+        ctx = ctx or default_context()
+        _bind_stream(ctx, xyz)
         rc = fn(ctx.handle, ctypes.byref(pts), tg, float(eps), int(terms), int(tree_depth), int(leaf_max), MEM_DEVICE,
                 flags, ctypes.c_void_p(pot.data_ptr()), ctypes.c_void_p(field.data_ptr()), ctypes.byref(rep))
         _check(rc, name)
         return pot, field, rep.as_dict()
-    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, dim)
-    q = np.ascontiguousarray(q, dtype=np.float64)
-    n = len(q)
+    if out is not None:
+        raise ValueError("out= takes CUDA tensors (device inputs); host calls return new arrays")
+    xyz = _host_points("positions", xyz, dim)
+    n = len(xyz)
+    q = _host_charges("charges", q, n)
     pts = Points3d(n, xyz.ctypes.data, dim, q.ctypes.data, 1)
     tg = None
     m = n
     if targets is not None:
-        targets = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, dim)
+        if _is_torch_cuda(targets):
+            raise TypeError("targets are a CUDA tensor but the sources are host arrays")
+        targets = _host_points("targets", targets, dim)
         m = len(targets)
         tg = ctypes.byref(Points3d(m, targets.ctypes.data, dim, None, 1))
     pot = np.empty(m)
     field = np.empty((m, dim))
+    ctx = ctx or default_context()
     rc = fn(ctx.handle, ctypes.byref(pts), tg, float(eps), int(terms), int(tree_depth), int(leaf_max), MEM_HOST, flags,
             pot.ctypes.data, field.ctypes.data, ctypes.byref(rep))
     _check(rc, name)
@@ -277,62 +356,64 @@ _KIND_WIDTH = {POTENTIAL2D: 1, FORCE2D: 2, POTENTIAL3D: 1, FORCE3D: 3}
 
 
 def _direct(dim, xyz, q, targets, exclude_self, ctx):
-    ctx = ctx or default_context()
     fn = lib().lfmm_direct3d if dim == 3 else lib().lfmm_direct2d
     flags = FLAG_EXCLUDE_SELF if exclude_self else 0
     if _is_torch_cuda(xyz):
         import torch
-        if xyz.dtype != torch.float64 or q.dtype != torch.float64 or xyz.stride(-1) != 1:
-            raise TypeError("device inputs must be float64, contiguous per point")
-        pts = Points3d(xyz.shape[0], xyz.data_ptr(), xyz.stride(0), q.data_ptr(), q.stride(0))
-        tg, m = None, xyz.shape[0]
+        n = _dev_points("positions", xyz, dim, xyz.device)
+        _dev_vector("charges", q, n, xyz.device)
+        pts = Points3d(n, xyz.data_ptr(), xyz.stride(0), q.data_ptr(), q.stride(0))
+        tg, m = None, n
         if targets is not None:
-            if targets.dtype != torch.float64 or targets.stride(-1) != 1:
-                raise TypeError("device targets must be float64, contiguous per point")
-            m = targets.shape[0]
+            m = _dev_points("targets", targets, dim, xyz.device)
             tg = ctypes.byref(Points3d(m, targets.data_ptr(), targets.stride(0), None, 1))
         pot = torch.empty(m, dtype=torch.float64, device=xyz.device)
         field = torch.empty((m, dim), dtype=torch.float64, device=xyz.device)
+        ctx = ctx or default_context()
+        _bind_stream(ctx, xyz)
         _check(fn(ctx.handle, ctypes.byref(pts), tg, MEM_DEVICE, flags, ctypes.c_void_p(pot.data_ptr()),
                   ctypes.c_void_p(field.data_ptr())), fn.__name__)
         return pot, field
-    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, dim)
-    q = np.ascontiguousarray(q, dtype=np.float64)
+    xyz = _host_points("positions", xyz, dim)
+    q = _host_charges("charges", q, len(xyz))
     pts = Points3d(len(q), xyz.ctypes.data, dim, q.ctypes.data, 1)
     tg, m = None, len(q)
     if targets is not None:
-        targets = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, dim)
+        targets = _host_points("targets", targets, dim)
         m = len(targets)
         tg = ctypes.byref(Points3d(m, targets.ctypes.data, dim, None, 1))
     pot = np.empty(m)
     field = np.empty((m, dim))
+    ctx = ctx or default_context()
     _check(fn(ctx.handle, ctypes.byref(pts), tg, MEM_HOST, flags, pot.ctypes.data, field.ctypes.data), fn.__name__)
     return pot, field
 
 
 def _vec_call(fn, name, xyz, qvec, targets, ctx, extra):
-    ctx = ctx or default_context()
     if _is_torch_cuda(xyz):
         import torch
-        if xyz.dtype != torch.float64 or qvec.dtype != torch.float64 or xyz.stride(-1) != 1 or qvec.stride(-1) != 1:
-            raise TypeError("device inputs must be float64, contiguous per point")
-        pts = Points3d(xyz.shape[0], xyz.data_ptr(), xyz.stride(0), qvec.data_ptr(), qvec.stride(0))
-        tg, m = None, xyz.shape[0]
+        n = _dev_points("positions", xyz, 3, xyz.device)
+        _dev_vector("charge vectors", qvec, n, xyz.device, 3)
+        pts = Points3d(n, xyz.data_ptr(), xyz.stride(0), qvec.data_ptr(), qvec.stride(0))
+        tg, m = None, n
         if targets is not None:
-            m = targets.shape[0]
+            m = _dev_points("targets", targets, 3, xyz.device)
             tg = ctypes.byref(Points3d(m, targets.data_ptr(), targets.stride(0), None, 1))
         force = torch.empty((m, 3), dtype=torch.float64, device=xyz.device)
+        ctx = ctx or default_context()
+        _bind_stream(ctx, xyz)
         _check(fn(ctx.handle, ctypes.byref(pts), tg, *extra(MEM_DEVICE, ctypes.c_void_p(force.data_ptr()))), name)
         return force
-    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
-    qvec = np.ascontiguousarray(qvec, dtype=np.float64).reshape(-1, 3)
+    xyz = _host_points("positions", xyz, 3)
+    qvec = _host_charges("charge vectors", qvec, len(xyz), 3)
     pts = Points3d(len(qvec), xyz.ctypes.data, 3, qvec.ctypes.data, 3)
     tg, m = None, len(qvec)
     if targets is not None:
-        targets = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+        targets = _host_points("targets", targets, 3)
         m = len(targets)
         tg = ctypes.byref(Points3d(m, targets.ctypes.data, 3, None, 1))
     force = np.empty((m, 3))
+    ctx = ctx or default_context()
     _check(fn(ctx.handle, ctypes.byref(pts), tg, *extra(MEM_HOST, force.ctypes.data)), name)
     return force
 
@@ -358,7 +439,6 @@ def fmm3d_batch(xyz, q, offsets, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, 
     Returns (pot (total,), field (total,3)[, list of report dicts]) in the same
     rows.  float64 numpy arrays take the host path, CUDA torch tensors the
     device path (outputs are then CUDA tensors, or written into out)."""
-    ctx = ctx or default_context()
     off = np.ascontiguousarray(offsets, dtype=np.int64)
     n_sys = len(off) - 1
     if n_sys < 0:
@@ -367,24 +447,34 @@ def fmm3d_batch(xyz, q, offsets, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, 
     reps = (Report * max(n_sys, 1))() if reports else None
     if _is_torch_cuda(xyz):
         import torch
-        if xyz.dtype != torch.float64 or q.dtype != torch.float64 or not xyz.is_contiguous() or not q.is_contiguous():
-            raise TypeError("device inputs must be contiguous float64")
-        total = xyz.shape[0]
+        total = _dev_points("positions", xyz, 3, xyz.device)
+        _dev_out("charges", q, total, xyz.device)
+        if total > 0 and not xyz.is_contiguous():
+            raise ValueError("positions must be contiguous (packed systems)")
         if out is None:
             pot = torch.empty(total, dtype=torch.float64, device=xyz.device)
             field = torch.empty((total, 3), dtype=torch.float64, device=xyz.device)
         else:
             pot, field = out
+            _dev_out("out potential", pot, total, xyz.device)
+            _dev_out("out field", field, total, xyz.device, 3)
+        ctx = ctx or default_context()
+        _bind_stream(ctx, xyz)
         args = (xyz.data_ptr(), q.data_ptr(), MEM_DEVICE, pot.data_ptr(), field.data_ptr())
     else:
-        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
-        q = np.ascontiguousarray(q, dtype=np.float64)
-        total = len(q)
+        if out is not None:
+            raise ValueError("out= takes CUDA tensors (device inputs); host calls return new arrays")
+        xyz = _host_points("positions", xyz, 3)
+        total = len(xyz)
+        q = _host_charges("charges", q, total)
         pot = np.empty(total)
         field = np.empty((total, 3))
         args = (xyz.ctypes.data, q.ctypes.data, MEM_HOST, pot.ctypes.data, field.ctypes.data)
     if n_sys > 0 and (off[0] != 0 or off[-1] != total):
         raise ValueError("offsets must run from 0 to the particle count")
+    if n_sys > 0 and np.any(np.diff(off) < 0):
+        raise ValueError("offsets must be non-decreasing")
+    ctx = ctx or default_context()
     rc = lib().lfmm_fmm3d_batch(ctx.handle, n_sys, off.ctypes.data, ctypes.c_void_p(args[0]), ctypes.c_void_p(args[1]),
                                 float(eps), int(terms), int(tree_depth), int(leaf_max), args[2], flags, int(workers),
                                 ctypes.c_void_p(args[3]), ctypes.c_void_p(args[4]),

@@ -338,7 +338,7 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
 
 int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, double eps, int terms_in,
                 int depth_in, int leaf_max, int memory, unsigned flags, double* pot_out, double* field_out,
-                lfmm_report* rep)
+                lfmm_report* rep, const DevTables* tables)
 {
     const auto T0 = std::chrono::steady_clock::now();
     const bool timed = (flags & LFMM_FLAG_TIMINGS) != 0;
@@ -370,15 +370,18 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     const double t_tree = timed ? ms_since(T0) : 0.0;
 
     // ---- expansions
-    const DevTables* DT = nullptr;
-    rc = get_tables(ctx, N, &DT);
-    if (rc != LFMM_OK) return rc;
+    const DevTables* DT = tables;
+    if (!DT) {
+        rc = get_tables(ctx, N, &DT);
+        if (rc != LFMM_OK) return rc;
+    }
     std::vector<double2*> outer(depth, nullptr), inner(depth, nullptr);
     const bool far = depth >= 3;
     size_t nev = 0;
     int m2l_launches = 0, mixed_launches = 0;
     std::vector<char> ev_mixed;  // per event pair: matrix-core M2L or not
     const bool mixed = far && (flags & LFMM_FLAG_MIXED) && depth - 1 >= kMixedFirstLevel;
+    if (mixed && tables) return LFMM_ERR_UNSUPPORTED;  // caller-placed tables hold no matrix-core operators
     uint4* xp = nullptr;
     uint32_t* xmax = nullptr;
     if (mixed) {
@@ -601,6 +604,41 @@ bool valid_points(const lfmm_points3d* p, bool need_q, int dim)
     return p->n < ((int64_t)1 << 31);
 }
 
+// Argument checks and the trivial cases of lfmm_fmm3d (nothing to evaluate;
+// no sources: zero outputs).  Returns kNeedSolve when a solve must run, with
+// *tg the target view (null for a single-set solve), else the final status.
+int fmm3d_prologue(lfmm_ctx* c, const lfmm_points3d* sources, const lfmm_points3d* targets, double eps, int terms,
+                   int tree_depth, int memory, double* potential, double* field, lfmm_report* report,
+                   const lfmm_points3d** tg_out)
+{
+    if (!c || !valid_points(sources, true, 3)) return LFMM_ERR_INVALID;
+    if (memory != LFMM_MEM_HOST && memory != LFMM_MEM_DEVICE) return LFMM_ERR_INVALID;
+    if (terms != LFMM_AUTO && terms < 1) return LFMM_ERR_TERMS;
+    if (tree_depth != LFMM_AUTO && (tree_depth < 1 || tree_depth > kMaxDepth)) return LFMM_ERR_DEPTH;
+    if (!(eps > 0.0) && terms == LFMM_AUTO) return LFMM_ERR_INVALID;
+    if (targets && !valid_points(targets, false, 3)) return LFMM_ERR_INVALID;
+    const lfmm_points3d* tg = (targets && targets->n > 0) ? targets : nullptr;
+    if (tg && sources->n + tg->n >= ((int64_t)1 << 31)) return LFMM_ERR_INVALID;
+    if (report) std::memset(report, 0, sizeof(*report));
+    *tg_out = tg;
+    if ((tg ? tg->n : sources->n) == 0) return LFMM_OK;
+    if (sources->n == 0) {  // no sources: every output is zero
+        std::lock_guard<std::mutex> g(c->mu);
+        if (hipSetDevice(c->device) != hipSuccess) return LFMM_ERR_HIP;
+        const size_t nt = (size_t)tg->n;
+        if (memory == LFMM_MEM_HOST) {
+            if (potential) std::memset(potential, 0, nt * sizeof(double));
+            if (field) std::memset(field, 0, 3 * nt * sizeof(double));
+        } else {
+            if (potential && fill_u32((uint32_t*)potential, 2 * (int64_t)nt, 0u, c->stream) != hipSuccess)
+                return LFMM_ERR_HIP;
+            if (field && fill_u32((uint32_t*)field, 6 * (int64_t)nt, 0u, c->stream) != hipSuccess) return LFMM_ERR_HIP;
+        }
+        return LFMM_OK;
+    }
+    return kNeedSolve;
+}
+
 static int tree_export(lfmm_ctx* c, const lfmm_points3d* particles, int dim, int tree_depth, int leaf_max,
                        int32_t* depth_out, int64_t* level_sizes, double* geom, int64_t* pos_map, uint64_t* keys,
                        int64_t* begins)
@@ -690,6 +728,8 @@ void lfmm_destroy(lfmm_ctx* c)
         (void)hipFree(kv.second.m16);
     }
     free_tables2d(c);
+    (void)hipFree(c->vec_tmp);
+    if (c->switch_event) (void)hipEventDestroy(c->switch_event);
     for (auto e : c->events) (void)hipEventDestroy(e);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
@@ -714,7 +754,18 @@ const char* lfmm_strerror(int s)
 int lfmm_set_stream(lfmm_ctx* c, void* s)
 {
     if (!c) return LFMM_ERR_INVALID;
-    c->stream = s ? (hipStream_t)s : c->own_stream;
+    std::lock_guard<std::mutex> g(c->mu);
+    hipStream_t ns = s ? (hipStream_t)s : c->own_stream;
+    if (ns == c->stream) return LFMM_OK;
+    // work already queued on the old stream (a device-memory solve returns
+    // before its kernels finish) comes before anything on the new one, which
+    // reuses the same workspace
+    if (hipSetDevice(c->device) != hipSuccess) return LFMM_ERR_HIP;
+    if (!c->switch_event && hipEventCreateWithFlags(&c->switch_event, hipEventDisableTiming) != hipSuccess)
+        return LFMM_ERR_HIP;
+    if (hipEventRecord(c->switch_event, c->stream) != hipSuccess) return LFMM_ERR_HIP;
+    if (hipStreamWaitEvent(ns, c->switch_event, 0) != hipSuccess) return LFMM_ERR_HIP;
+    c->stream = ns;
     return LFMM_OK;
 }
 
@@ -732,33 +783,14 @@ int lfmm_fmm3d(lfmm_ctx* c, const lfmm_points3d* sources, const lfmm_points3d* t
                int tree_depth, int leaf_max, int memory, unsigned flags, double* potential, double* field,
                lfmm_report* report)
 {
-    if (!c || !valid_points(sources, true, 3)) return LFMM_ERR_INVALID;
-    if (memory != LFMM_MEM_HOST && memory != LFMM_MEM_DEVICE) return LFMM_ERR_INVALID;
-    if (terms != LFMM_AUTO && terms < 1) return LFMM_ERR_TERMS;
-    if (tree_depth != LFMM_AUTO && (tree_depth < 1 || tree_depth > kMaxDepth)) return LFMM_ERR_DEPTH;
-    if (!(eps > 0.0) && terms == LFMM_AUTO) return LFMM_ERR_INVALID;
-    if (targets && !valid_points(targets, false, 3)) return LFMM_ERR_INVALID;
-    const lfmm_points3d* tg = (targets && targets->n > 0) ? targets : nullptr;
-    if (tg && sources->n + tg->n >= ((int64_t)1 << 31)) return LFMM_ERR_INVALID;
-    if (report) std::memset(report, 0, sizeof(*report));
-    if ((tg ? tg->n : sources->n) == 0) return LFMM_OK;
-    if (sources->n == 0) {  // no sources: every output is zero
-        std::lock_guard<std::mutex> g(c->mu);
-        if (hipSetDevice(c->device) != hipSuccess) return LFMM_ERR_HIP;
-        const size_t nt = (size_t)tg->n;
-        if (memory == LFMM_MEM_HOST) {
-            if (potential) std::memset(potential, 0, nt * sizeof(double));
-            if (field) std::memset(field, 0, 3 * nt * sizeof(double));
-        } else {
-            if (potential && hipMemsetAsync(potential, 0, nt * sizeof(double), c->stream) != hipSuccess) return LFMM_ERR_HIP;
-            if (field && hipMemsetAsync(field, 0, 3 * nt * sizeof(double), c->stream) != hipSuccess) return LFMM_ERR_HIP;
-        }
-        return LFMM_OK;
-    }
+    const lfmm_points3d* tg = nullptr;
+    const int pre = fmm3d_prologue(c, sources, targets, eps, terms, tree_depth, memory, potential, field, report, &tg);
+    if (pre != kNeedSolve) return pre;
     std::lock_guard<std::mutex> g(c->mu);
     if (hipSetDevice(c->device) != hipSuccess) return LFMM_ERR_HIP;
     try {
-        return fmm3d_solve(c, sources, tg, eps, terms, tree_depth, leaf_max, memory, flags, potential, field, report);
+        return fmm3d_solve(c, sources, tg, eps, terms, tree_depth, leaf_max, memory, flags, potential, field, report,
+                           nullptr);
     } catch (const std::bad_alloc&) {
         return LFMM_ERR_NOMEM;
     } catch (...) {

@@ -25,7 +25,6 @@ struct Arena {
     struct Chunk { char* p; size_t size; };
     std::vector<Chunk> chunks;
     size_t cur = 0, off = 0, used = 0, peak = 0;
-
     void reset()
     {
         if (chunks.size() > 1) {  // consolidate into one chunk sized for the last peak
@@ -98,6 +97,9 @@ struct lfmm_ctx {
     std::map<int, lfmm::host::DevTables> tables;
     std::map<int, lfmm::host::DevTables2d*> tables2d;
     std::vector<lfmm_ctx*> workers;  // lfmm_fmm3d_batch worker contexts, created on first use
+    hipEvent_t switch_event = nullptr;  // orders a stream switch (lfmm_set_stream)
+    void* vec_tmp = nullptr;            // two field buffers of the vector-charge device path
+    size_t vec_tmp_bytes = 0;
     std::mutex mu;
 };
 
@@ -154,6 +156,17 @@ int upload_view(lfmm_ctx* ctx, const lfmm_points3d* S, int dim, DevPoints& P);
 // The particle set of a solve: the sources alone, or [targets (q = 0); sources].
 int stage_inputs(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, int memory, int dim, DevPoints& P);
 int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, bool partition, Tree& T);
+// The 3D solve behind lfmm_fmm3d (arguments checked, ctx->mu held, device
+// set).  tables: operator tables already on the device, or null to take the
+// context's cached ones (built on first use).
+int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, double eps, int terms_in,
+                int depth_in, int leaf_max, int memory, unsigned flags, double* pot_out, double* field_out,
+                lfmm_report* rep, const DevTables* tables);
+int terms_3d(double eps, int terms);
+constexpr int kNeedSolve = -1;
+int fmm3d_prologue(lfmm_ctx* c, const lfmm_points3d* sources, const lfmm_points3d* targets, double eps, int terms,
+                   int tree_depth, int memory, double* potential, double* field, lfmm_report* report,
+                   const lfmm_points3d** tg_out);
 bool valid_points(const lfmm_points3d* p, bool need_q, int dim);
 void free_tables2d(lfmm_ctx* ctx);
 

@@ -63,9 +63,20 @@ int vec_solve(lfmm_ctx* c, const lfmm_points3d* sources, const lfmm_points3d* ta
     }
     double* tmp = nullptr;
     {
+        // the two component fields live in a per-context buffer that only
+        // grows (kept outside the workspace the scalar solves reset)
         std::lock_guard<std::mutex> g(c->mu);
         if (hipSetDevice(c->device) != hipSuccess) return LFMM_ERR_HIP;
-        if (hipMalloc(&tmp, 2 * bytes) != hipSuccess) return LFMM_ERR_NOMEM;
+        if (c->vec_tmp_bytes < 2 * bytes) {
+            // earlier work on the stream may still read the old buffer
+            if (c->vec_tmp && hipStreamSynchronize(c->stream) != hipSuccess) return LFMM_ERR_HIP;
+            (void)hipFree(c->vec_tmp);
+            c->vec_tmp = nullptr;
+            c->vec_tmp_bytes = 0;
+            if (hipMalloc(&c->vec_tmp, 2 * bytes) != hipSuccess) return LFMM_ERR_NOMEM;
+            c->vec_tmp_bytes = 2 * bytes;
+        }
+        tmp = (double*)c->vec_tmp;
     }
     const lfmm_points3d s0 = component(sources, 0), s1 = component(sources, 1), s2 = component(sources, 2);
     int rc = scalar(&s0, tg, tmp);
@@ -75,8 +86,7 @@ int vec_solve(lfmm_ctx* c, const lfmm_points3d* sources, const lfmm_points3d* ta
     if (hipSetDevice(c->device) != hipSuccess) rc = LFMM_ERR_HIP;
     if (rc == LFMM_OK && launch_cross_fields(tmp, tmp + 3 * n_out, force, n_out, c->stream) != hipSuccess)
         rc = LFMM_ERR_HIP;
-    if (rc == LFMM_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = LFMM_ERR_HIP;
-    (void)hipFree(tmp);
+    // device outputs follow the stream order of the context (no host wait)
     return rc;
 }
 
