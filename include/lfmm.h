@@ -64,6 +64,14 @@ typedef enum lfmm_memory {
                                       split FP16 (22-bit operands, FP32 sums);
                                       levels 2-3, L2L, P2M, M2M, L2P, P2P stay
                                       FP64.  Adds ~1e-7 relative error.      */
+#define LFMM_FLAG_ADAPTIVE 16u     /* 3D: adaptive octree.  A box is split only
+                                      while it holds more than leaf_max
+                                      particles (64 when leaf_max <= 0), down
+                                      to the uniform depth of that bound (or
+                                      tree_depth); leaves sit on any level and
+                                      the solve adds P2L (X lists) and M2P
+                                      (W lists) to M2L and P2P.  The reference
+                                      has uniform trees only.                 */
 
 typedef struct lfmm_ctx lfmm_ctx;
 
@@ -105,13 +113,20 @@ typedef struct lfmm_report {
     double t_leaf_kernel_ms;
     double t_m2l_mixed_ms;       /* matrix-core M2L kernels alone (timings)    */
     int64_t m2l_ops_mixed;       /* M2L translations on the matrix cores (count_ops) */
+    int64_t num_boxes;           /* boxes of all levels                        */
+    int64_t xw_pairs;            /* LFMM_FLAG_ADAPTIVE: X-list (P2L) pairs, equal
+                                    to the W-list (M2P) pairs                  */
 } lfmm_report;
 
 /* Create a context bound to a GPU (device index as HIP numbers them). */
 int lfmm_create(lfmm_ctx** ctx, int device);
 void lfmm_destroy(lfmm_ctx* ctx);
 const char* lfmm_strerror(int status);
-/* Use a caller-owned hipStream_t (NULL restores the context's own stream). */
+/* Use a caller-owned hipStream_t (NULL restores the context's own stream;
+ * LFMM_STREAM_DEFAULT selects the device's null stream, the handle 0 that
+ * e.g. torch reports for its default stream).  Work queued on the previous
+ * stream comes before anything the context queues afterwards. */
+#define LFMM_STREAM_DEFAULT ((void*)1)
 int lfmm_set_stream(lfmm_ctx* ctx, void* hip_stream);
 /* Release cached device workspace (it is otherwise reused across calls). */
 int lfmm_trim(lfmm_ctx* ctx);
@@ -157,6 +172,27 @@ int lfmm_fmm3d(lfmm_ctx* ctx, const lfmm_points3d* sources, const lfmm_points3d*
 int lfmm_tree3d(lfmm_ctx* ctx, const lfmm_points3d* particles, int tree_depth, int leaf_max,
                 int32_t* depth_out, int64_t* level_sizes, double* geom, int64_t* pos_map,
                 uint64_t* keys, int64_t* begins);
+
+/*
+ * The adaptive octree of a 3D particle set and its interaction lists, as
+ * lfmm_fmm3d builds them with LFMM_FLAG_ADAPTIVE (no reference counterpart:
+ * the reference's MortonTree, src/cpu/morton_tree.h:24-45, is uniform-depth).
+ * Host pointers.  counts[4] receives {boxes, leaves, U entries, X pairs};
+ * every other output may be NULL (call once with NULLs to size the arrays;
+ * the build is deterministic).  Boxes are numbered level by level, each level
+ * in key order; leaves in particle (Morton) order.
+ *   box_level [boxes], box_key [boxes], box_range [2 * boxes] (sorted
+ *   particle range), leaf_box [leaves] (box of each leaf);
+ *   U: u_off [leaves + 1], u_leaf [U entries]   leaves touching each leaf
+ *   X: x_off [boxes + 1],  x_leaf [X pairs]     coarser leaves whose particles
+ *                                               feed the box's local (P2L)
+ *   W: w_off [leaves + 1], w_box [X pairs]      finer boxes whose multipoles
+ *                                               act on the leaf (M2P)
+ */
+int lfmm_tree3d_adaptive(lfmm_ctx* ctx, const lfmm_points3d* particles, int tree_depth, int leaf_max,
+                         int64_t* counts, int32_t* depth_out, double* geom, int32_t* box_level, uint64_t* box_key,
+                         int32_t* box_range, int32_t* leaf_box, int64_t* u_off, int32_t* u_leaf, int64_t* x_off,
+                         int32_t* x_leaf, int64_t* w_off, int32_t* w_box);
 
 /*
  * 2D Laplace FMM with complex-series expansions; replaces the constructors of

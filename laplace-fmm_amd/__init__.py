@@ -22,7 +22,8 @@ CSRC = os.path.join(_HERE, "csrc")
 FMM_AUTO = 2147483647
 MAX_TERMS_3D = 18
 MEM_HOST, MEM_DEVICE = 0, 1
-FLAG_EXCLUDE_SELF, FLAG_TIMINGS, FLAG_COUNT_OPS, FLAG_MIXED = 1, 2, 4, 8
+FLAG_EXCLUDE_SELF, FLAG_TIMINGS, FLAG_COUNT_OPS, FLAG_MIXED, FLAG_ADAPTIVE = 1, 2, 4, 8, 16
+STREAM_DEFAULT = 1  # LFMM_STREAM_DEFAULT: the null stream
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "expansion terms out of range", 3: "tree depth out of range",
           4: "out of device memory", 5: "HIP runtime error", 6: "non-finite input", 7: "no usable GPU",
@@ -33,7 +34,7 @@ EXPORTS = ("lfmm_create", "lfmm_destroy", "lfmm_strerror", "lfmm_set_stream", "l
            "lfmm_tree3d", "lfmm_fmm2d", "lfmm_tree2d", "lfmm_version", "lfmm_host_wigner_d", "lfmm_host_op_tables",
            "lfmm_host_m2l_dense", "lfmm_host_m2l16_pack", "lfmm_direct3d", "lfmm_direct2d", "lfmm_exact_filename",
            "lfmm_write_exact", "lfmm_read_exact", "lfmm_error_stats", "lfmm_fmm3d_vec", "lfmm_direct3d_vec",
-           "lfmm_fmm3d_batch")
+           "lfmm_fmm3d_batch", "lfmm_tree3d_adaptive")
 
 
 class LfmmError(RuntimeError):
@@ -55,7 +56,7 @@ class Report(ctypes.Structure):
                 ("t_total_ms", ctypes.c_double), ("t_m2l_kernels_ms", ctypes.c_double),
                 ("m2l_launches", ctypes.c_int32), ("m2l_mixed_launches", ctypes.c_int32),
                 ("t_leaf_kernel_ms", ctypes.c_double), ("t_m2l_mixed_ms", ctypes.c_double),
-                ("m2l_ops_mixed", ctypes.c_int64)]
+                ("m2l_ops_mixed", ctypes.c_int64), ("num_boxes", ctypes.c_int64), ("xw_pairs", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -147,6 +148,9 @@ def lib():
         L.lfmm_direct3d_vec.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points3d), ctypes.POINTER(Points3d),
                                         ctypes.c_int, ctypes.c_void_p]
         L.lfmm_direct3d_vec.restype = ctypes.c_int
+        L.lfmm_tree3d_adaptive.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points3d), ctypes.c_int, ctypes.c_int] + \
+            [ctypes.c_void_p] * 13
+        L.lfmm_tree3d_adaptive.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -205,7 +209,10 @@ def _bind_stream(ctx, t):
     import torch
     if t.device.index is not None and t.device.index != ctx.device:
         raise ValueError("tensors are on %s but the context is bound to GPU %d" % (t.device, ctx.device))
-    ctx.set_stream(torch.cuda.current_stream(t.device).cuda_stream)
+    handle = torch.cuda.current_stream(t.device).cuda_stream
+    # torch's default stream is the null stream (handle 0); a NULL argument
+    # would select the context's own stream instead
+    ctx.set_stream(handle if handle else STREAM_DEFAULT)
 
 
 def _dev_points(name, t, dim, device):
@@ -273,11 +280,12 @@ def _host_charges(name, a, n, width=None):
 
 
 def _solve(dim, xyz, q, eps, terms, tree_depth, leaf_max, exclude_self, timings, count_ops, ctx, out, targets,
-           mixed):
+           mixed, adaptive=False):
     fn = lib().lfmm_fmm3d if dim == 3 else lib().lfmm_fmm2d
     name = "lfmm_fmm3d" if dim == 3 else "lfmm_fmm2d"
     flags = ((FLAG_EXCLUDE_SELF if exclude_self else 0) | (FLAG_TIMINGS if timings else 0) |
-             (FLAG_COUNT_OPS if count_ops else 0) | (FLAG_MIXED if mixed else 0))
+             (FLAG_COUNT_OPS if count_ops else 0) | (FLAG_MIXED if mixed else 0) |
+             (FLAG_ADAPTIVE if adaptive else 0))
     rep = Report()
     if _is_torch_cuda(xyz):
         import torch
@@ -326,7 +334,7 @@ def _solve(dim, xyz, q, eps, terms, tree_depth, leaf_max, exclude_self, timings,
 
 
 def fmm3d(xyz, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exclude_self=False,
-          timings=False, count_ops=False, ctx=None, out=None, targets=None, mixed=False):
+          timings=False, count_ops=False, ctx=None, out=None, targets=None, mixed=False, adaptive=False):
     """3D FMM: potential (m,) and field (m,3) at every target.
 
     xyz (n,3) and q (n,) are the sources; targets (m,3) defaults to the
@@ -335,9 +343,11 @@ def fmm3d(xyz, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exc
     written into `out=(pot, field)`).  Returns (pot, field, report_dict).
     mixed=True runs the M2L of levels >= 4 on the matrix cores in split FP16
     with FP32 accumulation (LFMM_FLAG_MIXED); everything else stays FP64.
+    adaptive=True builds the adaptive octree (LFMM_FLAG_ADAPTIVE): boxes of
+    more than leaf_max particles (64 if 0) are split, leaves on any level.
     """
     return _solve(3, xyz, q, eps, terms, tree_depth, leaf_max, exclude_self, timings, count_ops, ctx, out, targets,
-                  mixed)
+                  mixed, adaptive)
 
 
 def fmm2d(xy, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exclude_self=False,
@@ -566,6 +576,36 @@ def _tree(dim, xyz, q, tree_depth, leaf_max, ctx):
         out["begin"].append(ranges[b:b + s + 1].copy())
         o += s
         b += s + 1
+    return out
+
+
+def tree3d_adaptive(xyz, leaf_max=64, tree_depth=FMM_AUTO, ctx=None):
+    """The adaptive octree and its U, X, W lists (lfmm_tree3d_adaptive) as a
+    dict of numpy arrays: depth, geom, box_level, box_key, box_range (sorted
+    particle ranges), leaf_box, u_off / u_leaf, x_off / x_leaf, w_off / w_box."""
+    xyz = _host_points("positions", xyz, 3)
+    ctx = ctx or default_context()
+    pts = Points3d(len(xyz), xyz.ctypes.data, 3, None, 1)
+    counts = np.zeros(4, np.int64)
+    nul = [None] * 13
+    nul[0] = counts.ctypes.data
+    _check(lib().lfmm_tree3d_adaptive(ctx.handle, ctypes.byref(pts), int(tree_depth), int(leaf_max), *nul),
+           "lfmm_tree3d_adaptive")
+    nb, nl, nu, nx = (int(v) for v in counts)
+    out = dict(depth=np.zeros(1, np.int32), geom=np.zeros(4), box_level=np.zeros(nb, np.int32),
+               box_key=np.zeros(nb, np.uint64), box_range=np.zeros((nb, 2), np.int32), leaf_box=np.zeros(nl, np.int32),
+               u_off=np.zeros(nl + 1, np.int64), u_leaf=np.zeros(max(nu, 1), np.int32),
+               x_off=np.zeros(nb + 1, np.int64), x_leaf=np.zeros(max(nx, 1), np.int32),
+               w_off=np.zeros(nl + 1, np.int64), w_box=np.zeros(max(nx, 1), np.int32))
+    order = ("depth", "geom", "box_level", "box_key", "box_range", "leaf_box", "u_off", "u_leaf", "x_off", "x_leaf",
+             "w_off", "w_box")
+    _check(lib().lfmm_tree3d_adaptive(ctx.handle, ctypes.byref(pts), int(tree_depth), int(leaf_max),
+                                      counts.ctypes.data, *[out[k].ctypes.data for k in order]),
+           "lfmm_tree3d_adaptive")
+    out["depth"] = int(out["depth"][0])
+    out["u_leaf"] = out["u_leaf"][:nu]
+    out["x_leaf"] = out["x_leaf"][:nx]
+    out["w_box"] = out["w_box"][:nx]
     return out
 
 

@@ -196,7 +196,8 @@ int stage_inputs(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg,
 // MortonTree constructor (src/cpu/morton_tree.cpp:286-493) on the GPU.
 // `partition` applies the q = 0 partition of every leaf (morton_tree.cpp:426-468);
 // a single-set solve skips it, since it evaluates every particle either way.
-int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, bool partition, Tree& T)
+int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, bool partition, Tree& T,
+               bool keep_parents)
 {
     const int64_t n = DP.n;
     const int dim = DP.dim;
@@ -317,14 +318,19 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
         if (!T.src) return LFMM_ERR_NOMEM;
         LFMM_HIP(gather_sources(dim, dpos, ps, dq, qs, order, n, T.src, st));
     }
+    if (keep_parents) T.uparent.assign(depth, nullptr);
     for (int l = leaf_level - 1; l >= 0; --l) {
         const int64_t m = T.cnt[l + 1];
         const int64_t cap = std::min<int64_t>(m, (int64_t)1 << (dim * l));
         T.lkeys[l] = take<uint64_t>(A, cap);
         T.lbegin[l] = take<int32_t>(A, cap + 1);
         if (!T.lkeys[l] || !T.lbegin[l]) return LFMM_ERR_NOMEM;
-        LFMM_HIP(run_compact(T.lkeys[l + 1], m, dim, flags_ws, excl_ws, scan_ws, T.lkeys[l], T.lbegin[l], nullptr,
-                             &T.cnt[l], st));
+        if (keep_parents) {
+            T.uparent[l + 1] = take<int32_t>(A, m);
+            if (!T.uparent[l + 1]) return LFMM_ERR_NOMEM;
+        }
+        LFMM_HIP(run_compact(T.lkeys[l + 1], m, dim, flags_ws, excl_ws, scan_ws, T.lkeys[l], T.lbegin[l],
+                             keep_parents ? T.uparent[l + 1] : nullptr, &T.cnt[l], st));
     }
     for (int l = 2; l < depth; ++l) {
         if (l > lut_max_level(dim)) continue;
@@ -354,8 +360,17 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     if (rc != LFMM_OK) return rc;
     const int64_t n = DP.n;
     Tree T;
-    rc = build_tree(ctx, DP, depth_in, leaf_max, Tg != nullptr, T);
+    const bool adaptive = (flags & LFMM_FLAG_ADAPTIVE) != 0;
+    // the adaptive octree splits boxes of more than leaf_max particles (64
+    // when not given), down to the depth at which every box holds that many
+    if (adaptive && leaf_max <= 0) leaf_max = 64;
+    rc = build_tree(ctx, DP, depth_in, leaf_max, Tg != nullptr, T, adaptive);
     if (rc != LFMM_OK) return rc;
+    if (adaptive) {
+        rc = build_adaptive(ctx, leaf_max, T);
+        if (rc != LFMM_OK) return rc;
+        LFMM_HIP(max_run(T.leaf_begin, T.nleaf, T.umax, &T.max_leaf, st));
+    }
     const int depth = T.depth;
     const int leaf_level = depth - 1;
     const double* geom = T.geom;
@@ -367,6 +382,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     std::vector<int32_t*>& lbegin = T.lbegin;
     std::vector<int32_t*>& lut = T.lut;
     const uint32_t max_leaf = T.max_leaf;
+    const int64_t nleaf = adaptive ? T.nleaf : cnt[leaf_level];
     const double t_tree = timed ? ms_since(T0) : 0.0;
 
     // ---- expansions
@@ -398,8 +414,21 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         if (!xp || !xmax) return LFMM_ERR_NOMEM;
         LFMM_HIP(fill_u32(xmax, depth, 0u, st));
     }
+    // adaptive octree: the expansions of all boxes in one array each, by gid
+    double2* outer_all = nullptr;
+    double2* inner_all = nullptr;
     if (far) {
+        if (adaptive) {
+            outer_all = take<double2>(A, (int64_t)T.total() * NX);
+            inner_all = take<double2>(A, (int64_t)T.total() * NX);
+            if (!outer_all || !inner_all) return LFMM_ERR_NOMEM;
+        }
         for (int l = 2; l < depth; ++l) {
+            if (adaptive) {
+                outer[l] = outer_all + (int64_t)T.loff[l] * NX;
+                inner[l] = inner_all + (int64_t)T.loff[l] * NX;
+                continue;
+            }
             outer[l] = take<double2>(A, cnt[l] * NX);
             inner[l] = take<double2>(A, cnt[l] * NX);
             if (!outer[l] || !inner[l]) return LFMM_ERR_NOMEM;
@@ -408,6 +437,15 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                    outer[leaf_level]};
         // the matrix-core levels need their largest |coefficient|: the producers max-combine it
         if (mixed && leaf_level >= kMixedFirstLevel) pa.absmax = xmax + leaf_level;
+        if (adaptive) {  // every leaf of the global list, at its level
+            pa.leaf_begin = T.leaf_begin;
+            pa.leaf_key = T.leaf_key;
+            pa.nleaf = (int)T.nleaf;
+            pa.outer = outer_all;
+            pa.leaf_gid = T.leaf_gid;
+            pa.leaf_lvl = T.leaf_lvl;
+            pa.absmax = mixed ? xmax : nullptr;
+        }
         LFMM_HIP(launch_p2m(N, pa, st));
         for (int l = leaf_level - 1; l >= 2; --l) {
             RtrLevel L{};
@@ -496,6 +534,15 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                 ev_mixed.push_back(0);
             }
             ++m2l_launches;
+            if (adaptive && T.xseg[l + 1] > T.xseg[l]) {
+                // X lists of the level: source particles into the locals (P2L)
+                P2MArgs px{src, nullptr, T.keys_all, (int)(T.xseg[l + 1] - T.xseg[l]), l, geom, DT->knm, inner_all};
+                px.unit_box = T.xlist + T.xseg[l];
+                px.x_off = T.x_off;
+                px.x_rng = T.x_rng;
+                px.g2 = T.loff[2];
+                LFMM_HIP(launch_p2l(N, px, st));
+            }
         }
     }
     double t_down = 0.0;
@@ -517,14 +564,28 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     la.leaf_of = leaf_of;
     la.leaf_key = lkeys[leaf_level];
     la.leaf_begin = lbegin[leaf_level];
-    la.nleaf = (int)cnt[leaf_level];
+    la.nleaf = (int)nleaf;
     la.level = leaf_level;
     la.geom = geom;
     la.lut = leaf_level <= lut_max_level(3) ? (leaf_level >= 2 ? lut[leaf_level] : nullptr) : nullptr;
     la.local = far ? inner[leaf_level] : nullptr;
     la.knm = DT->knm;
     la.perm = order;
-    {
+    if (adaptive) {
+        la.leaf_key = T.leaf_key;
+        la.leaf_begin = T.leaf_begin;
+        la.local = inner_all;
+        la.nbr = T.u_rng;
+        la.nbr_off = T.u_off;
+        la.leaf_lvl = T.leaf_lvl;
+        la.leaf_gid = T.leaf_gid;
+        la.w_off = T.w_off;
+        la.w_gid = T.w_gid;
+        la.keys_all = T.keys_all;
+        la.outer_all = outer_all;
+        la.depth = depth;
+        for (int l = 0; l <= depth; ++l) la.loff[l] = T.loff[l];
+    } else {
         int2* nbr = take<int2>(A, cnt[leaf_level] * 27);
         if (!nbr) return LFMM_ERR_NOMEM;
         LFMM_HIP(launch_leaf_ranges(lkeys[leaf_level], lbegin[leaf_level], (int)cnt[leaf_level], leaf_level, la.lut, nbr, st));
@@ -539,8 +600,8 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         la.far = take<double4>(A, n);
         if (!la.far) return LFMM_ERR_NOMEM;
     }
-    int32_t* p2p_ws = take<int32_t>(A, p2p_ws_elems(n, cnt[leaf_level]));
-    uint32_t* p2p_scan = take<uint32_t>(A, scan_ws_elems(cnt[leaf_level]));
+    int32_t* p2p_ws = take<int32_t>(A, p2p_ws_elems(n, nleaf));
+    uint32_t* p2p_scan = take<uint32_t>(A, scan_ws_elems(nleaf));
     if (!p2p_ws || !p2p_scan) return LFMM_ERR_NOMEM;
     const size_t ev_leaf = nev;
     if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
@@ -556,7 +617,10 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     if (rep) {
         rep->terms = N;
         rep->tree_depth = depth;
-        rep->num_leaves = cnt[leaf_level];
+        rep->num_leaves = nleaf;
+        rep->num_boxes = 0;
+        for (int l = 0; l < depth; ++l) rep->num_boxes += cnt[l];
+        rep->xw_pairs = adaptive ? T.x_pairs : 0;
         rep->max_leaf = max_leaf;
         rep->m2l_ops = -1;
         rep->m2l_ops_mixed = -1;
@@ -587,6 +651,10 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             if (!cws) return LFMM_ERR_NOMEM;
             std::vector<int64_t> per(depth, 0);
             LFMM_HIP(count_ops(3, lkeys, lbegin, lut, cnt, cws, &rep->m2l_ops, &rep->p2p_pairs, per.data(), st));
+            if (adaptive) {
+                rc = ad_count_pairs(T.leaf_begin, T.u_off, T.u_rng, T.nleaf, cws, &rep->p2p_pairs, st);
+                if (rc != LFMM_OK) return rc;
+            }
             rep->m2l_ops_mixed = 0;
             if (mixed)
                 for (int l = kMixedFirstLevel; l < depth; ++l) rep->m2l_ops_mixed += per[l];
@@ -689,6 +757,94 @@ static int tree_export(lfmm_ctx* c, const lfmm_points3d* particles, int dim, int
     }
 }
 
+template <typename T>
+static int fetch(std::vector<T>& h, const T* d, int64_t count, hipStream_t st)
+{
+    h.resize((size_t)std::max<int64_t>(count, 0));
+    if (count > 0) LFMM_HIP(hipMemcpyAsync(h.data(), d, sizeof(T) * (size_t)count, hipMemcpyDeviceToHost, st));
+    return LFMM_OK;
+}
+
+static int tree_export_adaptive(lfmm_ctx* c, const lfmm_points3d* particles, int tree_depth, int leaf_max,
+                                int64_t* counts, int32_t* depth_out, double* geom, int32_t* box_level,
+                                uint64_t* box_key, int32_t* box_range, int32_t* leaf_box, int64_t* u_off,
+                                int32_t* u_leaf, int64_t* x_off, int32_t* x_leaf, int64_t* w_off, int32_t* w_box)
+{
+    if (!c || !valid_points(particles, false, 3) || particles->n == 0 || !counts) return LFMM_ERR_INVALID;
+    if (tree_depth != LFMM_AUTO && (tree_depth < 1 || tree_depth > kMaxDepth)) return LFMM_ERR_DEPTH;
+    if (leaf_max <= 0) leaf_max = 64;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return LFMM_ERR_HIP;
+    try {
+        c->arena.reset();
+        Tree T;
+        DevPoints DP;
+        int rc = upload_view(c, particles, 3, DP);
+        if (rc != LFMM_OK) return rc;
+        rc = build_tree(c, DP, tree_depth, leaf_max, false, T, true);
+        if (rc != LFMM_OK) return rc;
+        rc = build_adaptive(c, leaf_max, T);
+        if (rc != LFMM_OK) return rc;
+        hipStream_t st = c->stream;
+        const int total = T.total(), g2 = T.depth > 2 ? T.loff[2] : total;
+        std::vector<uint64_t> keys;
+        std::vector<int2> rng, urng, xrng;
+        std::vector<int32_t> lb, lg, wg, b2l;
+        std::vector<uint32_t> uo, xo, wo;
+        if ((rc = fetch(keys, T.keys_all, total, st)) || (rc = fetch(rng, T.range_all, total, st)) ||
+            (rc = fetch(lb, T.leaf_begin, T.nleaf + 1, st)) || (rc = fetch(lg, T.leaf_gid, T.nleaf, st)) ||
+            (rc = fetch(uo, T.u_off, T.nleaf + 1, st)) || (rc = fetch(urng, T.u_rng, T.u_total, st)) ||
+            (rc = fetch(wo, T.w_off, T.nleaf + 1, st)) || (rc = fetch(wg, T.w_gid, T.x_pairs, st)) ||
+            (rc = fetch(b2l, T.box2leaf, total, st)))
+            return rc;
+        if (total > g2) {
+            if ((rc = fetch(xo, T.x_off, total - g2 + 1, st)) || (rc = fetch(xrng, T.x_rng, T.x_pairs, st))) return rc;
+        }
+        LFMM_HIP(hipStreamSynchronize(st));
+        counts[0] = total;
+        counts[1] = T.nleaf;
+        counts[2] = T.u_total;
+        counts[3] = T.x_pairs;
+        if (depth_out) *depth_out = T.depth;
+        if (geom) LFMM_HIP(hipMemcpy(geom, T.geom, 4 * sizeof(double), hipMemcpyDeviceToHost));
+        // a particle range of a leaf -> the leaf's number
+        auto leaf_at = [&](int b) {
+            return (int32_t)(std::upper_bound(lb.begin(), lb.end() - 1, b) - lb.begin() - 1);
+        };
+        for (int gi = 0; gi < total; ++gi) {
+            int l = 0;
+            while (l + 1 < T.depth && gi >= T.loff[l + 1]) ++l;
+            if (box_level) box_level[gi] = l;
+            if (box_key) box_key[gi] = keys[gi];
+            if (box_range) {
+                box_range[2 * gi] = rng[gi].x;
+                box_range[2 * gi + 1] = rng[gi].y;
+            }
+        }
+        for (int64_t i = 0; i < T.nleaf; ++i) {
+            if (leaf_box) leaf_box[i] = lg[i];
+            if (u_off) u_off[i] = uo[i];
+            if (w_off) w_off[i] = wo[i];
+        }
+        if (u_off) u_off[T.nleaf] = uo[T.nleaf];
+        if (w_off) w_off[T.nleaf] = wo[T.nleaf];
+        if (u_leaf)
+            for (int64_t k = 0; k < T.u_total; ++k) u_leaf[k] = leaf_at(urng[k].x);
+        if (w_box)
+            for (int64_t k = 0; k < T.x_pairs; ++k) w_box[k] = wg[k];
+        if (x_off) {
+            for (int gi = 0; gi <= total; ++gi) x_off[gi] = gi < g2 ? 0 : (int64_t)xo[gi - g2];
+        }
+        if (x_leaf)
+            for (int64_t k = 0; k < T.x_pairs; ++k) x_leaf[k] = leaf_at(xrng[k].x);
+        return LFMM_OK;
+    } catch (const std::bad_alloc&) {
+        return LFMM_ERR_NOMEM;
+    } catch (...) {
+        return LFMM_ERR_INVALID;
+    }
+}
+
 }  // namespace lfmm::host
 
 using namespace lfmm;
@@ -755,7 +911,8 @@ int lfmm_set_stream(lfmm_ctx* c, void* s)
 {
     if (!c) return LFMM_ERR_INVALID;
     std::lock_guard<std::mutex> g(c->mu);
-    hipStream_t ns = s ? (hipStream_t)s : c->own_stream;
+    // LFMM_STREAM_DEFAULT: the null stream (handle 0), which a NULL argument cannot name
+    hipStream_t ns = s == LFMM_STREAM_DEFAULT ? (hipStream_t)0 : (s ? (hipStream_t)s : c->own_stream);
     if (ns == c->stream) return LFMM_OK;
     // work already queued on the old stream (a device-memory solve returns
     // before its kernels finish) comes before anything on the new one, which
@@ -802,6 +959,15 @@ int lfmm_tree3d(lfmm_ctx* c, const lfmm_points3d* particles, int tree_depth, int
                 int64_t* level_sizes, double* geom, int64_t* pos_map, uint64_t* keys, int64_t* begins)
 {
     return tree_export(c, particles, 3, tree_depth, leaf_max, depth_out, level_sizes, geom, pos_map, keys, begins);
+}
+
+int lfmm_tree3d_adaptive(lfmm_ctx* c, const lfmm_points3d* particles, int tree_depth, int leaf_max, int64_t* counts,
+                         int32_t* depth_out, double* geom, int32_t* box_level, uint64_t* box_key, int32_t* box_range,
+                         int32_t* leaf_box, int64_t* u_off, int32_t* u_leaf, int64_t* x_off, int32_t* x_leaf,
+                         int64_t* w_off, int32_t* w_box)
+{
+    return tree_export_adaptive(c, particles, tree_depth, leaf_max, counts, depth_out, geom, box_level, box_key,
+                                box_range, leaf_box, u_off, u_leaf, x_off, x_leaf, w_off, w_box);
 }
 
 int lfmm_tree2d(lfmm_ctx* c, const lfmm_points2d* particles, int tree_depth, int leaf_max, int32_t* depth_out,

@@ -5,6 +5,7 @@
 #include "lfmm_internal.h"
 #include "ops3d.h"
 #include "tree.h"
+#include "adaptive.h"
 
 #include <hip/hip_runtime.h>
 
@@ -136,6 +137,33 @@ struct Tree {
     std::vector<int32_t*> lbegin;  // leaves: particle ranges; above: child ranges
     std::vector<int32_t*> lut;
     uint32_t max_leaf = 0;
+    // uniform build with keep_parents: parent index of every box at levels >= 1
+    std::vector<int32_t*> uparent;
+    // ---- adaptive octree (build_adaptive).  cnt / lkeys / lbegin / lut then
+    // describe the kept boxes of each level (lbegin: child ranges, empty for
+    // a leaf); boxes are also numbered globally (gid), level by level.
+    bool adaptive = false;
+    std::vector<int> loff;            // gid of the first box of each level, depth + 1 entries
+    uint64_t* keys_all = nullptr;     // [total] (lkeys[l] = keys_all + loff[l])
+    int2* range_all = nullptr;        // [total] particle range of each box
+    int32_t* box2leaf = nullptr;      // [total] leaf index or -1
+    int64_t nleaf = 0;                // global leaf list, in particle order:
+    int32_t* leaf_begin = nullptr;    //   [nleaf + 1] first particle
+    int32_t* leaf_gid = nullptr;      //   [nleaf] box
+    uint64_t* leaf_key = nullptr;     //   [nleaf] key at its level
+    int32_t* leaf_lvl = nullptr;      //   [nleaf] level
+    uint32_t* u_off = nullptr;        // U lists: [nleaf + 1] offsets into u_rng
+    int2* u_rng = nullptr;            //   particle ranges of the touching leaves
+    int64_t u_total = 0;
+    uint32_t* x_off = nullptr;        // X lists: per box t = gid - loff[2], [loff[depth] - loff[2] + 1]
+    int2* x_rng = nullptr;            //   particle ranges of the source leaves
+    int32_t* xlist = nullptr;         //   boxes with a non-empty X list (t), gid order
+    std::vector<int64_t> xseg;        //   xlist entries of level l: [xseg[l], xseg[l+1])
+    int64_t x_pairs = 0;
+    uint32_t* w_off = nullptr;        // W lists: [nleaf + 1] offsets into w_gid
+    int32_t* w_gid = nullptr;         //   boxes whose multipoles act on the leaf
+    int total() const { return loff.empty() ? 0 : loff.back(); }
+    AdLevels view() const;
 };
 
 // Device view of the particle set a solve works on.
@@ -155,7 +183,14 @@ double ms_since(std::chrono::steady_clock::time_point t0);
 int upload_view(lfmm_ctx* ctx, const lfmm_points3d* S, int dim, DevPoints& P);
 // The particle set of a solve: the sources alone, or [targets (q = 0); sources].
 int stage_inputs(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, int memory, int dim, DevPoints& P);
-int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, bool partition, Tree& T);
+int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, bool partition, Tree& T,
+               bool keep_parents = false);
+// Turns the uniform tree T (built with keep_parents) into the adaptive octree
+// of leaf size leaf_max and builds its U, X and W lists (adaptive.hip).
+int build_adaptive(lfmm_ctx* ctx, int leaf_max, Tree& T);
+// Near-field pair count of an adaptive tree's U lists (ws: 1 counter).  Synchronises.
+int ad_count_pairs(const int32_t* leaf_begin, const uint32_t* u_off, const int2* u_rng, int64_t nleaf,
+                   unsigned long long* ws, int64_t* pairs, hipStream_t st);
 // The 3D solve behind lfmm_fmm3d (arguments checked, ctx->mu held, device
 // set).  tables: operator tables already on the device, or null to take the
 // context's cached ones (built on first use).

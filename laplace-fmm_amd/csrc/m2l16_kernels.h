@@ -101,7 +101,8 @@ template <int N>
 __global__ void __launch_bounds__(256) k_m2l16_pack_cls(const double2* __restrict__ outer, int num_parents,
                                                         const int32_t* __restrict__ child_begin,
                                                         const uint64_t* __restrict__ child_keys,
-                                                        const uint32_t* __restrict__ xmax, uint4* __restrict__ xp)
+                                                        const uint32_t* __restrict__ xmax, uint4* __restrict__ xp,
+                                                        int64_t stride)
 {
     using S = M16<N>;
     const int64_t np1 = (int64_t)num_parents + 1;
@@ -121,7 +122,7 @@ __global__ void __launch_bounds__(256) k_m2l16_pack_cls(const double2* __restric
         const int child = (j < ce && (int)(child_keys[j] & 7u) == c) ? j++ : -1;
         m16_h8 hv = {}, lv = {};
         if (child >= 0) {
-            const double* src = (const double*)(outer + (int64_t)child * tri(N));
+            const double* src = (const double*)(outer + (int64_t)child * stride);
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
                 const int k = kc * 16 + half * 8 + s;
@@ -443,9 +444,10 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
     uint32_t tx[CTW], ty[CTW], tz[CTW];
 #pragma unroll
     for (int ct = 0; ct < CTW; ++ct) {
-        const int P = tile * TC + 32 * (CTW * w + ct) + (lane & 31);
+        const int Pi = tile * TC + 32 * (CTW * w + ct) + (lane & 31);
         int t = -1;
-        if (P < a.num_parents) {
+        if (Pi < (a.plist ? a.plist_count : a.num_parents)) {
+            const int P = a.plist ? a.plist[Pi] : Pi;
             const int cb = a.child_begin[P], ce = a.child_begin[P + 1];
             for (int i = cb; i < ce; ++i)
                 if ((int)(a.child_keys[i] & 7u) == c) t = i;
@@ -662,10 +664,11 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
     // which have no real coordinate, as zeros)
     const double inv = a.kinv / m16_xscale(*a.xmax);
     double2* const dst = a.dsplit > 1 ? a.partial + (int64_t)part * a.size * tri(N) : a.dst;
+    const int64_t SX = (a.dsplit == 1 && a.stride > 0) ? a.stride : tri(N);
 #pragma unroll
     for (int ct = 0; ct < CTW; ++ct) {
         if (tgt[ct] < 0) continue;
-        double* out = (double*)(dst + (int64_t)tgt[ct] * tri(N));
+        double* out = (double*)(dst + (int64_t)tgt[ct] * SX);
         if (a.dsplit > 1) {
             // lanes l and l + 32 hold the same target
             for (int n = lane >> 5; n < N; n += 2) out[2 * tri(n) + 1] = 0.0;
@@ -696,7 +699,8 @@ hipError_t do_m2l16_pack(const M2L16Args& a, hipStream_t st)
     if constexpr (m16_rowfull(N)) {
         const int64_t packs = ((int64_t)a.num_parents + 1) * M16<N>::KC * 2;
         hipLaunchKernelGGL((k_m2l16_pack_cls<N>), dim3((unsigned)((packs + 255) / 256)), dim3(256), 0, st, a.outer,
-                           a.num_parents, a.child_begin, a.child_keys, a.xmax, a.xp);
+                           a.num_parents, a.child_begin, a.child_keys, a.xmax, a.xp,
+                           (int64_t)(a.stride > 0 ? a.stride : tri(N)));
     } else {
         const int64_t packs = (int64_t)(a.size + 1) * M16<N>::KC * 2;
         hipLaunchKernelGGL((k_m2l16_pack<N>), dim3((unsigned)((packs + 255) / 256)), dim3(256), 0, st, a.outer,
@@ -709,8 +713,10 @@ template <int N>
 hipError_t do_m2l16(const M2L16Args& a, hipStream_t st)
 {
     static_assert(m16_targets(N) == (m16_rowfull(N) ? 128 * m16r_ctw(N) : 32 * m16_ct(N)), "tile width");
-    const int tiles = (a.num_parents + m16_targets(N) - 1) / m16_targets(N);
+    const int targets = a.plist ? a.plist_count : a.num_parents;
+    const int tiles = (targets + m16_targets(N) - 1) / m16_targets(N);
     if (tiles != a.tiles || a.dsplit < 1 || a.dsplit > kDirsPerClass) return hipErrorInvalidValue;
+    if (a.plist && (a.dsplit != 1 || !m16_rowfull(N))) return hipErrorInvalidValue;
     if (tiles > 0) {
         if constexpr (m16_rowfull(N))
             hipLaunchKernelGGL((k_m2l16r<N, m16r_ctw(N)>), dim3(tiles * 8 * a.dsplit), dim3(256), 0, st, a, a.kmat,

@@ -5,6 +5,7 @@
 namespace lfmm {
 
 template <int N> hipError_t do_p2m(const P2MArgs& a, hipStream_t st);
+template <int N> hipError_t do_p2l(const P2MArgs& a, hipStream_t st);
 template <int N> hipError_t do_rtr(int kind, const RtrTables& T, const RtrLevel& L, hipStream_t st);
 template <int N> hipError_t do_l2p(const LeafArgs& a, hipStream_t st);
 template <int N> hipError_t do_m2l16_pack(const M2L16Args& a, hipStream_t st);
@@ -16,6 +17,16 @@ hipError_t launch_p2m(int N, const P2MArgs& a, hipStream_t st)
 {
     switch (N) {
 #define X(n) case n: return do_p2m<n>(a, st);
+        LFMM_FOR_TERMS(X)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_p2l(int N, const P2MArgs& a, hipStream_t st)
+{
+    switch (N) {
+#define X(n) case n: return do_p2l<n>(a, st);
         LFMM_FOR_TERMS(X)
 #undef X
     default: return hipErrorInvalidValue;

@@ -15,6 +15,18 @@ struct P2MArgs {
     const double* knm;         // K_nm table, tri(N+1)
     double2* outer;            // normalised multipoles of the leaves
     uint32_t* absmax = nullptr;   // if set: max-combined largest |coefficient| (float bits)
+    // adaptive octree, P2M: unit i is leaf i of the global leaf list, written
+    // to outer + leaf_gid[i] * NX; leaf_lvl[i] its level (absmax + level)
+    const int32_t* leaf_gid = nullptr;
+    const int32_t* leaf_lvl = nullptr;
+    // adaptive octree, P2L (launch_p2l): unit i is box g2 + unit_box[i] of
+    // `level` (key leaf_key[g2 + t], keys by gid); its sources are the
+    // particle ranges x_rng[x_off[t] .. x_off[t + 1]); the result is added to
+    // outer + (g2 + t) * NX (the locals)
+    const int32_t* unit_box = nullptr;
+    const uint32_t* x_off = nullptr;
+    const int2* x_rng = nullptr;
+    int g2 = 0;
 };
 
 struct RtrTables {
@@ -53,6 +65,11 @@ struct RtrLevel {
     int dsplit = 1;
     double2* partial = nullptr;
     uint32_t* absmax = nullptr;   // M2M: if set, max-combined largest |coefficient| written (float bits)
+    // per-box orders: the parents are plist[0 .. num_parents) (indices at
+    // level-1) and every expansion array has `stride` coefficients per box
+    // (0: tri(N)); requires dsplit == 1
+    const int32_t* plist = nullptr;
+    int stride = 0;
 };
 
 struct LeafArgs {
@@ -75,6 +92,21 @@ struct LeafArgs {
     double* field;
     double4* far = nullptr;    // with locals: per sorted particle (phi, field) of L2P, written by
                                // k_l2p and read by the pair kernel (split launch)
+    // adaptive octree: leaves on several levels.  leaf_key / leaf_begin /
+    // leaf_of then describe the global leaf list; leaf_lvl its levels; local
+    // the locals of all boxes by gid (leaf_gid); nbr / nbr_off the U lists as
+    // CSR of particle ranges; w_off / w_gid the W lists, whose multipoles
+    // (outer_all, keys_all by gid, level offsets loff) act on the leaf's
+    // particles (M2P, in k_l2p)
+    const int32_t* leaf_lvl = nullptr;
+    const int32_t* leaf_gid = nullptr;
+    const uint32_t* nbr_off = nullptr;
+    const uint32_t* w_off = nullptr;
+    const int32_t* w_gid = nullptr;
+    const uint64_t* keys_all = nullptr;
+    const double2* outer_all = nullptr;
+    int depth = 0;
+    int loff[17] = {};
 };
 
 // Matrix-core M2L of one level (m2l16_kernels.h).
@@ -101,6 +133,12 @@ struct M2L16Args {
     int tiles = 0;             // target tiles of 32 * m16 column tiles
     int dsplit = 1;            // direction parts (partial + p*size*tri(N) when > 1)
     double2* partial = nullptr;
+    // per-box orders: target parents plist[0 .. tiles' parents) (indices at
+    // level-1; the packing still covers all num_parents) and `stride`
+    // coefficients per box in outer and dst (0: tri(N)); needs dsplit == 1
+    const int32_t* plist = nullptr;
+    int plist_count = 0;
+    int stride = 0;
 };
 
 // Matrix-core M2L kernel per expansion size: the row-complete k_m2l16r
@@ -119,6 +157,8 @@ constexpr int m16_targets(int N)
 }
 
 hipError_t launch_p2m(int N, const P2MArgs& a, hipStream_t st);
+// P2L of one level's X lists (adaptive octree; see P2MArgs)
+hipError_t launch_p2l(int N, const P2MArgs& a, hipStream_t st);
 hipError_t launch_rtr(int N, int kind, const RtrTables& T, const RtrLevel& L, hipStream_t st);
 // L2P alone into a.far (k_l2p), for the pair kernels below
 hipError_t launch_l2p(int N, const LeafArgs& a, hipStream_t st);

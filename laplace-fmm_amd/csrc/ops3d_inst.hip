@@ -9,6 +9,7 @@
 
 namespace lfmm {
 template hipError_t do_p2m<LFMM_N>(const P2MArgs&, hipStream_t);
+template hipError_t do_p2l<LFMM_N>(const P2MArgs&, hipStream_t);
 template hipError_t do_rtr<LFMM_N>(int, const RtrTables&, const RtrLevel&, hipStream_t);
 template hipError_t do_l2p<LFMM_N>(const LeafArgs&, hipStream_t);
 template hipError_t do_m2l16_pack<LFMM_N>(const M2L16Args&, hipStream_t);

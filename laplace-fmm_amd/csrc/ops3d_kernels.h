@@ -91,7 +91,17 @@ struct P2MOrder {  // position -> tri index
     }
 };
 
-template <int N>
+// Kinds of k_p2m units: the leaves of a uniform tree; the leaves of the
+// adaptive tree's global list (several levels); the boxes of one level's X
+// lists (P2L).
+enum { P2M_UNIFORM = 0, P2M_LEAVES = 1, P2M_P2L = 2 };
+
+// P2L by Kelvin inversion: with u the source position relative to the target
+// centre (in box units), Lt_nm = sum q K_nm conj(I_n^m(u)) where
+// I_n^m(u) = P_n^m(cos theta) e^{i m phi} / |u|^(n+1) is the irregular solid
+// harmonic, and I_n^m(u) = S_n^m(u / |u|^2) / |u|.  So the P2M recurrences
+// at u / |u|^2 with charge q / |u| give the local expansion.
+template <int N, int MODE>
 __global__ void __launch_bounds__(256) k_p2m(P2MArgs a, const double* __restrict__ knm)
 {
     constexpr int NX = tri(N);
@@ -100,35 +110,89 @@ __global__ void __launch_bounds__(256) k_p2m(P2MArgs a, const double* __restrict
     __shared__ double2 red[4][16][65];  // per wave [entry][lane], rows padded: conflict-free column reads
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int row = lane >> 4, slot = lane & 15;
-    const int leaf = (blockIdx.x * 4 + w) * 4 + row;
-    const bool ok = leaf < a.nleaf;
+    const int unit = (blockIdx.x * 4 + w) * 4 + row;
+    const bool ok = unit < a.nleaf;
     double cx = 0.0, cy = 0.0, cz = 0.0, ih = 0.0;
-    int b = 0, e = 0;
+    int b = 0, e = 0;      // particle range (P2L: ranges kb .. ke of x_rng)
+    int level = a.level;
+    int64_t box = unit;    // output expansion
     if (ok) {
-        const uint64_t key = a.leaf_key[leaf];
+        uint64_t key;
+        if constexpr (MODE == P2M_P2L) {
+            const int t = a.unit_box[unit];
+            box = a.g2 + t;
+            key = a.leaf_key[box];
+            b = (int)a.x_off[t];
+            e = (int)a.x_off[t + 1];
+        } else {
+            key = a.leaf_key[unit];
+            if constexpr (MODE == P2M_LEAVES) {
+                box = a.leaf_gid[unit];
+                level = a.leaf_lvl[unit];
+            }
+            b = a.leaf_begin[unit];
+            e = a.leaf_begin[unit + 1];
+        }
         const double scale = a.geom[3];
-        cx = box_centre(compact3(key >> 2), a.level, scale, a.geom[0]);
-        cy = box_centre(compact3(key >> 1), a.level, scale, a.geom[1]);
-        cz = box_centre(compact3(key), a.level, scale, a.geom[2]);
-        ih = ldexp(1.0, a.level) / scale;
-        b = a.leaf_begin[leaf];
-        e = a.leaf_begin[leaf + 1];
+        cx = box_centre(compact3(key >> 2), level, scale, a.geom[0]);
+        cy = box_centre(compact3(key >> 1), level, scale, a.geom[1]);
+        cz = box_centre(compact3(key), level, scale, a.geom[2]);
+        ih = ldexp(1.0, level) / scale;
     }
-    int rounds = (e - b + 15) >> 4;
+    // P2L: this lane walks the concatenated ranges from position `slot` in
+    // steps of 16 (cursor k, j); the row's rounds cover the total count
+    int k = b, j = 0, total = e - b;
+    auto settle = [&]() {
+        while (k < e) {
+            const int2 r = a.x_rng[k];
+            if (j < r.y) break;
+            const int over = j - r.y;
+            if (++k < e) j = a.x_rng[k].x + over;
+        }
+    };
+    if constexpr (MODE == P2M_P2L) {
+        total = 0;
+        for (int i = b; i < e; ++i) {
+            const int2 r = a.x_rng[i];
+            total += r.y - r.x;
+        }
+        if (b < e) {
+            j = a.x_rng[b].x + slot;
+            settle();
+        }
+    }
+    int rounds = (total + 15) >> 4;
     for (int off = 32; off > 0; off >>= 1) rounds = max(rounds, __shfl_xor(rounds, off));
     double2 (*R)[65] = red[w];
     double2 acc[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) acc[c] = make_double2(0.0, 0.0);
     for (int rd = 0; rd < rounds; ++rd) {
-        const int p = b + slot + 16 * rd;
         double ux = 0.0, uy = 0.0, uz = 0.0, q = 0.0;
-        if (p < e) {
+        int p = -1;
+        if constexpr (MODE == P2M_P2L) {
+            if (k < e) p = j;
+        } else {
+            p = b + slot + 16 * rd;
+            if (p >= e) p = -1;
+        }
+        if (p >= 0) {
             const double4 s4 = a.src[p];
             ux = (s4.x - cx) * ih;
             uy = (s4.y - cy) * ih;
             uz = (s4.z - cz) * ih;
             q = s4.w;
+            if constexpr (MODE == P2M_P2L) {
+                const double i2 = 1.0 / (ux * ux + uy * uy + uz * uz);
+                ux *= i2;
+                uy *= i2;
+                uz *= i2;
+                q *= sqrt(i2);
+            }
+        }
+        if constexpr (MODE == P2M_P2L) {
+            j += 16;
+            settle();
         }
         const double r2 = ux * ux + uy * uy + uz * uz;
         auto put = [&](auto PI, double2 S) {
@@ -142,8 +206,8 @@ __global__ void __launch_bounds__(256) k_p2m(P2MArgs a, const double* __restrict
                 double sx = 0.0, sy = 0.0;
                 if (slot <= (pos & 15)) {
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        const double2 v = R[slot][row * 16 + j];
+                    for (int jj = 0; jj < 16; ++jj) {
+                        const double2 v = R[slot][row * 16 + jj];
                         sx += v.x;
                         sy += v.y;
                     }
@@ -181,19 +245,36 @@ __global__ void __launch_bounds__(256) k_p2m(P2MArgs a, const double* __restrict
     }
     double mx = 0.0;
     if (ok) {
-        double2* out = a.outer + (int64_t)leaf * NX;
+        double2* out = a.outer + box * NX;
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             const int pos = c * 16 + slot;
             if (pos < NX) {
-                const double k = knm[ORD.idx[pos]];
-                const double2 v = make_double2(k * acc[c].x, k * acc[c].y);
-                out[ORD.idx[pos]] = v;
+                const double kk = knm[ORD.idx[pos]];
+                const double2 v = make_double2(kk * acc[c].x, kk * acc[c].y);
+                if constexpr (MODE == P2M_P2L) {
+                    const double2 o = out[ORD.idx[pos]];
+                    out[ORD.idx[pos]] = make_double2(o.x + v.x, o.y + v.y);
+                } else {
+                    out[ORD.idx[pos]] = v;
+                }
                 mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
             }
         }
     }
-    if (a.absmax) wave_absmax(mx, a.absmax);
+    if constexpr (MODE == P2M_UNIFORM) {
+        if (a.absmax) wave_absmax(mx, a.absmax);
+    } else if constexpr (MODE == P2M_LEAVES) {
+        // the leaves of a wave may sit on different levels: one atomic per row
+        if (a.absmax) {
+            for (int off = 8; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+            if (ok && slot == 0) {
+                uint32_t* dst = a.absmax + level;
+                const uint32_t bits = __float_as_uint((float)mx);
+                if (bits > __atomic_load_n(dst, __ATOMIC_RELAXED)) atomicMax(dst, bits);
+            }
+        }
+    }
 }
 
 // ===================================================================== rotate-translate-rotate
@@ -295,8 +376,14 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
         tile = blk;
         c = 0;
     }
-    const int P = tile * TW + lane;
-    const bool pvalid = lok && P < L.num_parents;
+    // parent of this lane: the next of the level, or of the caller's list
+    // (per-box orders: the parents whose children take this launch's N)
+    const int Pi = tile * TW + lane;
+    const bool pvalid = lok && Pi < L.num_parents;
+    const int P = (pvalid && L.plist) ? L.plist[Pi] : Pi;
+    // coefficient stride of the expansion arrays (larger than tri(N) when
+    // the boxes of a level carry different orders)
+    const int64_t SX = L.stride > 0 ? L.stride : NX;
     int cb = 0, ce = 0;
     if (pvalid) {
         cb = L.child_begin[P];
@@ -314,7 +401,8 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
             tz = compact3(k);
         }
     } else {
-        tgt = pvalid ? P : -1;
+        // a parent without children (a leaf of the adaptive tree) keeps its P2M
+        tgt = (pvalid && ce > cb) ? P : -1;
     }
     const int side = 1 << L.level;
     double2 acc[NX];
@@ -358,7 +446,7 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
     // the phase-A source coefficients of the next step, loaded one step ahead
     double2 pf[PF.count > 0 ? PF.count : 1];
     auto prefetch = [&](const RtrStep& s) {
-        const double2* sp = s.src + (int64_t)(s.sidx < 0 ? 0 : s.sidx) * NX;
+        const double2* sp = s.src + (int64_t)(s.sidx < 0 ? 0 : s.sidx) * SX;
         sfor<0, N>([&](auto NN) {
             constexpr int n = decltype(NN)::value;
             if constexpr (PT.deg[n] == W) {
@@ -459,7 +547,10 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
     }
     double mx = 0.0;
     if (tgt >= 0) {
-        double2* out = (L.dsplit > 1 ? L.partial + (int64_t)part * L.size * NX : L.dst) + (int64_t)tgt * NX;
+        double2* out = L.dsplit > 1 ? L.partial + ((int64_t)part * L.size + tgt) * NX : L.dst + (int64_t)tgt * SX;
+        // coefficients beyond this launch's order are zero
+        if constexpr (W == 0)
+            for (int64_t i = NX; i < SX; ++i) out[i] = make_double2(0.0, 0.0);
         sfor<0, N>([&](auto JJ) {
             constexpr int j = decltype(JJ)::value;
             if constexpr (PT.deg[j] == W) {
@@ -577,7 +668,13 @@ __device__ __forceinline__ void l2p_eval(const double2* __restrict__ Lt, const d
 // L2P alone: the far field of every evaluated particle in sorted order.  Run
 // before the pair kernel, which then starts from it, so that the pair loop
 // is not held to the register budget (and occupancy) of the L2P recurrences.
-template <int N>
+// ADAPTIVE: leaves sit on several levels (leaf_lvl; no local below level 2),
+// and the multipoles of the leaf's W list are evaluated at the particle too
+// (M2P).  M2P by Kelvin inversion: with u the particle relative to the source
+// centre (box units), r = |u| and f, g = l2p_eval of the multipole at
+// u / r^2, the potential is f / r and its gradient
+// (g - 2 (u/r)((u/r).g) - f u) / r^3 (I_n^m(u) = S_n^m(u / r^2) / r).
+template <int N, bool ADAPTIVE>
 __global__ void __launch_bounds__(256) k_l2p(LeafArgs a, const double2* __restrict__ local, const double* __restrict__ knm)
 {
     const int64_t s = xcd_local_block(blockIdx.x, gridDim.x) * (int64_t)blockDim.x + threadIdx.x;
@@ -585,17 +682,54 @@ __global__ void __launch_bounds__(256) k_l2p(LeafArgs a, const double2* __restri
     if (a.perm[s] >= a.n_out) return;
     const int leaf = a.leaf_of[s];
     const uint64_t key = a.leaf_key[leaf];
-    const uint32_t ix = compact3(key >> 2), iy = compact3(key >> 1), iz = compact3(key);
+    const int level = ADAPTIVE ? a.leaf_lvl[leaf] : a.level;
     const double4 t = a.src[s];
     const double scale = a.geom[3];
-    const double cx = box_centre(ix, a.level, scale, a.geom[0]);
-    const double cy = box_centre(iy, a.level, scale, a.geom[1]);
-    const double cz = box_centre(iz, a.level, scale, a.geom[2]);
-    const double ih = ldexp(1.0, a.level) / scale;
-    double lp, gx, gy, gz;
-    l2p_eval<N>(local + (int64_t)leaf * tri(N), knm, (t.x - cx) * ih, (t.y - cy) * ih, (t.z - cz) * ih, lp, gx, gy, gz);
-    const double ih2 = ih * ih;
-    a.far[s] = make_double4(lp * ih, -gx * ih2, -gy * ih2, -gz * ih2);
+    double fp = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    if (!ADAPTIVE || level >= 2) {
+        const uint32_t ix = compact3(key >> 2), iy = compact3(key >> 1), iz = compact3(key);
+        const double cx = box_centre(ix, level, scale, a.geom[0]);
+        const double cy = box_centre(iy, level, scale, a.geom[1]);
+        const double cz = box_centre(iz, level, scale, a.geom[2]);
+        const double ih = ldexp(1.0, level) / scale;
+        const int64_t box = ADAPTIVE ? a.leaf_gid[leaf] : leaf;
+        double lp, gx, gy, gz;
+        l2p_eval<N>(local + box * tri(N), knm, (t.x - cx) * ih, (t.y - cy) * ih, (t.z - cz) * ih, lp, gx, gy, gz);
+        const double ih2 = ih * ih;
+        fp = lp * ih;
+        fx = -gx * ih2;
+        fy = -gy * ih2;
+        fz = -gz * ih2;
+    }
+    if constexpr (ADAPTIVE) {
+        const uint32_t wb = a.w_off[leaf], we = a.w_off[leaf + 1];
+        for (uint32_t k = wb; k < we; ++k) {
+            const int g = a.w_gid[k];
+            int lv = 0;
+            while (lv + 1 < a.depth && g >= a.loff[lv + 1]) ++lv;
+            const uint64_t sk = a.keys_all[g];
+            const double ih = ldexp(1.0, lv) / scale;
+            const double ux = (t.x - box_centre(compact3(sk >> 2), lv, scale, a.geom[0])) * ih;
+            const double uy = (t.y - box_centre(compact3(sk >> 1), lv, scale, a.geom[1])) * ih;
+            const double uz = (t.z - box_centre(compact3(sk), lv, scale, a.geom[2])) * ih;
+            const double i2 = 1.0 / (ux * ux + uy * uy + uz * uz);
+            const double ir = sqrt(i2);
+            double f, gx, gy, gz;
+            l2p_eval<N>(a.outer_all + (int64_t)g * tri(N), knm, ux * i2, uy * i2, uz * i2, f, gx, gy, gz);
+            // gradient in u of f(u / r^2) / r
+            const double ud = (ux * gx + uy * gy + uz * gz) * i2;  // (u.g) / r^2
+            const double ir3 = i2 * ir;
+            const double dx = (gx - 2.0 * ux * ud - f * ux) * ir3;
+            const double dy = (gy - 2.0 * uy * ud - f * uy) * ir3;
+            const double dz = (gz - 2.0 * uz * ud - f * uz) * ir3;
+            fp += f * ir * ih;
+            const double ih2 = ih * ih;
+            fx -= dx * ih2;
+            fy -= dy * ih2;
+            fz -= dz * ih2;
+        }
+    }
+    a.far[s] = make_double4(fp, fx, fy, fz);
 }
 
 // ===================================================================== launchers
@@ -603,7 +737,18 @@ template <int N>
 hipError_t do_p2m(const P2MArgs& a, hipStream_t st)
 {
     const int nb = (a.nleaf + 15) / 16;  // 4 waves x 4 leaves
-    if (nb > 0) hipLaunchKernelGGL(k_p2m<N>, dim3(nb), dim3(256), 0, st, a, a.knm);
+    if (nb > 0) {
+        if (a.leaf_gid) hipLaunchKernelGGL((k_p2m<N, P2M_LEAVES>), dim3(nb), dim3(256), 0, st, a, a.knm);
+        else hipLaunchKernelGGL((k_p2m<N, P2M_UNIFORM>), dim3(nb), dim3(256), 0, st, a, a.knm);
+    }
+    return hipGetLastError();
+}
+
+template <int N>
+hipError_t do_p2l(const P2MArgs& a, hipStream_t st)
+{
+    const int nb = (a.nleaf + 15) / 16;
+    if (nb > 0) hipLaunchKernelGGL((k_p2m<N, P2M_P2L>), dim3(nb), dim3(256), 0, st, a, a.knm);
     return hipGetLastError();
 }
 
@@ -629,7 +774,10 @@ template <int N>
 hipError_t do_l2p(const LeafArgs& a, hipStream_t st)
 {
     const int nb = (int)((a.n + 255) / 256);
-    if (nb > 0) hipLaunchKernelGGL((k_l2p<N>), dim3(nb), dim3(256), 0, st, a, a.local, a.knm);
+    if (nb > 0) {
+        if (a.leaf_lvl) hipLaunchKernelGGL((k_l2p<N, true>), dim3(nb), dim3(256), 0, st, a, a.local, a.knm);
+        else hipLaunchKernelGGL((k_l2p<N, false>), dim3(nb), dim3(256), 0, st, a, a.local, a.knm);
+    }
     return hipGetLastError();
 }
 

@@ -32,17 +32,35 @@ __device__ __forceinline__ double rsqrt_nf(double r2)
     return y * fma(-hr * y, y, 1.5);
 }
 
-// cnt[leaf]: lanes of the leaf; total[leaf]: sources in its 27 neighbour ranges
+// Near-source ranges of a leaf: 27 per leaf (uniform tree), or the CSR U list
+// of the adaptive tree (nbr_off).
+__device__ inline void leaf_ranges(const int2* nbr, const uint32_t* nbr_off, int leaf, const int2*& nr, int& cnt)
+{
+    if (nbr_off) {
+        const uint32_t b = nbr_off[leaf];
+        nr = nbr + b;
+        cnt = (int)(nbr_off[leaf + 1] - b);
+    } else {
+        nr = nbr + (int64_t)leaf * 27;
+        cnt = 27;
+    }
+}
+
+// cnt[leaf]: lanes of the leaf; total[leaf]: sources in its near ranges
 template <int TPL>
 __global__ void __launch_bounds__(256) k_pair_counts(const int32_t* __restrict__ leaf_begin, const int2* __restrict__ nbr,
-                                                    int nleaf, uint32_t* __restrict__ cnt, int32_t* __restrict__ total)
+                                                    const uint32_t* __restrict__ nbr_off, int nleaf,
+                                                    uint32_t* __restrict__ cnt, int32_t* __restrict__ total)
 {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nleaf) return;
     cnt[l] = (uint32_t)((leaf_begin[l + 1] - leaf_begin[l] + TPL - 1) / TPL);
+    const int2* nr;
+    int nc;
+    leaf_ranges(nbr, nbr_off, l, nr, nc);
     int t = 0;
-    for (int k = 0; k < 27; ++k) {
-        const int2 r = nbr[(int64_t)l * 27 + k];
+    for (int k = 0; k < nc; ++k) {
+        const int2 r = nr[k];
         t += max(r.y - r.x, 0);
     }
     total[l] = t;
@@ -89,21 +107,24 @@ __global__ void __launch_bounds__(256) k_p2p_multi(LeafArgs a, const int32_t* __
             fz[u] = f.w;
         }
     }
-    const int2* __restrict__ nr = a.nbr + (int64_t)leaf * 27;
-    // the 27 ranges walked as one list of total[leaf] sources, the next
+    const int2* nr;
+    int nrc;
+    leaf_ranges(a.nbr, a.nbr_off, leaf, nr, nrc);
+    const int kl = nrc - 1;  // every list holds the leaf itself: nrc >= 1
+    // the near ranges walked as one list of total[leaf] sources, the next
     // range's bounds loaded ahead: a wave runs as long as its longest leaf
     // total, not as the sum over the ranges of its longest leaf in each.  The
     // loop runs on a plain count (the range advance never leaves it), which
     // keeps the accumulators in place across iterations.
     int k = 0;
-    int2 r = nr[0], rn = nr[1];
+    int2 r = nr[0], rn = nr[min(1, kl)];
     int j = r.x;
     const int nsrc = near_total[leaf];
     for (int it = 0; it < nsrc; ++it) {
-        while (j >= r.y && k < 26) {  // total counts these ranges: k stays <= 26
+        while (j >= r.y && k < kl) {  // total counts these ranges: k stays <= kl
             ++k;
             r = rn;
-            rn = nr[min(k + 1, 26)];
+            rn = nr[min(k + 1, kl)];
             j = r.x;
         }
         const double4 p = a.src[min(j++, (int)a.n - 1)];
@@ -159,7 +180,7 @@ hipError_t launch_p2p(const LeafArgs& a, int32_t* ws, uint32_t* scan_ws, hipStre
     int32_t* total = (int32_t*)(slot + a.nleaf);
     int32_t* lanes = total + a.nleaf;
     hipLaunchKernelGGL((k_pair_counts<kTPL>), dim3(nblk(a.nleaf, 256)), dim3(256), 0, st, a.leaf_begin, a.nbr,
-                       a.nleaf, cnt, total);
+                       a.nbr_off, a.nleaf, cnt, total);
     hipError_t e = exclusive_scan(cnt, slot, a.nleaf, scan_ws, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_pair_lanes<kTPL>), dim3(nblk(a.n, 256)), dim3(256), 0, st, a.leaf_of, a.leaf_begin, slot,
