@@ -73,6 +73,15 @@ typedef enum lfmm_memory {
                                       (W lists) to M2L and P2P.  The reference
                                       has uniform trees only.                 */
 
+#define LFMM_FLAG_LEARNED_ORDER 32u /* 3D: per-box expansion orders chosen by
+                                      the context's order network
+                                      (lfmm_set_order_model) on the adaptive
+                                      octree (implies LFMM_FLAG_ADAPTIVE).  A
+                                      sampled FP64 direct sum checks every run;
+                                      above the model's tolerance the solve is
+                                      rerun with p = 16 (report->fallback).
+                                      terms and eps are then ignored.        */
+
 typedef struct lfmm_ctx lfmm_ctx;
 
 /* Strided particle view.  Particle i has position
@@ -116,6 +125,16 @@ typedef struct lfmm_report {
     int64_t num_boxes;           /* boxes of all levels                        */
     int64_t xw_pairs;            /* LFMM_FLAG_ADAPTIVE: X-list (P2L) pairs, equal
                                     to the W-list (M2P) pairs                  */
+    /* Per-box orders (LFMM_FLAG_LEARNED_ORDER, lfmm_fmm3d_orders): terms is
+     * the largest order of any box, terms_min the smallest leaf order. */
+    int32_t terms_min;
+    int32_t fallback;            /* 1: the learned orders missed the sampled
+                                    check and the solve was rerun at p = 16   */
+    double check_rel_l2;         /* sampled check of a learned-order solve:   */
+    double check_rel_l2_field;   /* rel-L2 of potential and field vs FP64 direct
+                                    sums (self pair excluded)                 */
+    double m2l_flops;            /* with LFMM_FLAG_COUNT_OPS: algorithmic FP64
+                                    flops of all M2L operations at their orders */
 } lfmm_report;
 
 /* Create a context bound to a GPU (device index as HIP numbers them). */
@@ -172,6 +191,65 @@ int lfmm_fmm3d(lfmm_ctx* ctx, const lfmm_points3d* sources, const lfmm_points3d*
 int lfmm_tree3d(lfmm_ctx* ctx, const lfmm_points3d* particles, int tree_depth, int leaf_max,
                 int32_t* depth_out, int64_t* level_sizes, double* geom, int64_t* pos_map,
                 uint64_t* keys, int64_t* begins);
+
+/*
+ * Per-box expansion orders (no reference counterpart: the reference uses one
+ * expansion size per solve, src/cpu/multipole3d.cpp:1328-1331).
+ *
+ * lfmm_fmm3d_orders: lfmm_fmm3d on the adaptive octree with the order of
+ *   every leaf set to the largest request of its particles:
+ *   particle_terms[i] (terms N = p + 1 for target i: n_out entries, memory as
+ *   the outputs), clamped to [1, 18].  A box with children takes the largest
+ *   order of its children; the M2L and L2L into the children of a parent use
+ *   the parent's order from level 4 on (levels 2-3 use the largest order).
+ *   report->terms is the largest order, report->terms_min the smallest leaf
+ *   order.
+ *
+ * The order network: 8 per-leaf features -> 64 -> 64 -> 13 classes (FP32),
+ * class c = order p = 4 + c.  Its parameter block holds LFMM_ORDER_MODEL_SIZE
+ * floats: feature normalisation mu[8], sd[8], then W1[64][8], b1[64],
+ * W2[64][64], b2[64], W3[13][64], b3[13] (row-major, out x in).  Features of a
+ * leaf: log2(1 + particles), level / 16, log10 sum|q|, log10(|dipole| /
+ * (|monopole| + |dipole|)) about the leaf centre in box units, V-list size /
+ * 189, log10(V sources' sum|q| / the leaf's), log2(1 + near-field
+ * particles), log2(1 + |W| + |X|).  A leaf takes the smallest order whose
+ * cumulative class probability reaches `confidence`.
+ *
+ * lfmm_set_order_model: copies the model to the device (NULL removes it);
+ *   check_tol: sampled rel-L2 potential error above which a learned-order
+ *   solve is rerun at p = 16; check_samples targets (evenly spaced in input
+ *   order) are summed directly.
+ * lfmm_order_features: the adaptive octree of a particle set (as built by a
+ *   solve with the same tree_depth and leaf_max) with every leaf's features
+ *   (features [leaves * 8]) and every particle's leaf (particle_leaf [n],
+ *   input order, leaves numbered in Morton order).  *nleaf receives the leaf
+ *   count; call with NULL arrays to size them.
+ * lfmm_order_predict: class probabilities (probs [n * 13]) and/or chosen
+ *   terms (terms [n]) of n feature rows.
+ * lfmm_order_loss_grad: mean cross-entropy of n labelled rows (labels =
+ *   classes 0..12) and its gradient (grad [LFMM_ORDER_MODEL_SIZE]; zero for
+ *   the normalisation entries).
+ * lfmm_order_train: Adam on the mean cross-entropy, `epochs` passes over the
+ *   rows in batches of `batch`, shuffled by `seed`; model is updated in
+ *   place; loss_history (optional) receives the mean loss of each epoch.
+ * All order-network arrays are host pointers; the work runs on the GPU.
+ */
+#define LFMM_ORDER_FEATURES 8
+#define LFMM_ORDER_CLASSES 13
+#define LFMM_ORDER_MIN_TERMS 5
+#define LFMM_ORDER_MODEL_SIZE 5597
+int lfmm_fmm3d_orders(lfmm_ctx* ctx, const lfmm_points3d* sources, const lfmm_points3d* targets,
+                      const int32_t* particle_terms, int tree_depth, int leaf_max, int memory, unsigned flags,
+                      double* potential, double* field, lfmm_report* report);
+int lfmm_set_order_model(lfmm_ctx* ctx, const float* model, double confidence, double check_tol, int check_samples);
+int lfmm_order_features(lfmm_ctx* ctx, const lfmm_points3d* particles, int tree_depth, int leaf_max, int64_t* nleaf,
+                        float* features, int32_t* particle_leaf);
+int lfmm_order_predict(lfmm_ctx* ctx, const float* model, const float* features, int64_t n, double confidence,
+                       float* probs, int32_t* terms);
+int lfmm_order_loss_grad(lfmm_ctx* ctx, const float* model, const float* features, const int32_t* labels, int64_t n,
+                         float* loss, float* grad);
+int lfmm_order_train(lfmm_ctx* ctx, float* model, const float* features, const int32_t* labels, int64_t n,
+                     int epochs, int batch, double lr, uint64_t seed, float* loss_history);
 
 /*
  * The adaptive octree of a 3D particle set and its interaction lists, as

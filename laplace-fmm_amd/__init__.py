@@ -22,7 +22,8 @@ CSRC = os.path.join(_HERE, "csrc")
 FMM_AUTO = 2147483647
 MAX_TERMS_3D = 18
 MEM_HOST, MEM_DEVICE = 0, 1
-FLAG_EXCLUDE_SELF, FLAG_TIMINGS, FLAG_COUNT_OPS, FLAG_MIXED, FLAG_ADAPTIVE = 1, 2, 4, 8, 16
+FLAG_EXCLUDE_SELF, FLAG_TIMINGS, FLAG_COUNT_OPS, FLAG_MIXED, FLAG_ADAPTIVE, FLAG_LEARNED_ORDER = 1, 2, 4, 8, 16, 32
+ORDER_FEATURES, ORDER_CLASSES, ORDER_MIN_TERMS, ORDER_MODEL_SIZE = 8, 13, 5, 5597
 STREAM_DEFAULT = 1  # LFMM_STREAM_DEFAULT: the null stream
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "expansion terms out of range", 3: "tree depth out of range",
@@ -34,7 +35,8 @@ EXPORTS = ("lfmm_create", "lfmm_destroy", "lfmm_strerror", "lfmm_set_stream", "l
            "lfmm_tree3d", "lfmm_fmm2d", "lfmm_tree2d", "lfmm_version", "lfmm_host_wigner_d", "lfmm_host_op_tables",
            "lfmm_host_m2l_dense", "lfmm_host_m2l16_pack", "lfmm_direct3d", "lfmm_direct2d", "lfmm_exact_filename",
            "lfmm_write_exact", "lfmm_read_exact", "lfmm_error_stats", "lfmm_fmm3d_vec", "lfmm_direct3d_vec",
-           "lfmm_fmm3d_batch", "lfmm_tree3d_adaptive")
+           "lfmm_fmm3d_batch", "lfmm_tree3d_adaptive", "lfmm_fmm3d_orders", "lfmm_set_order_model",
+           "lfmm_order_features", "lfmm_order_predict", "lfmm_order_loss_grad", "lfmm_order_train")
 
 
 class LfmmError(RuntimeError):
@@ -56,7 +58,9 @@ class Report(ctypes.Structure):
                 ("t_total_ms", ctypes.c_double), ("t_m2l_kernels_ms", ctypes.c_double),
                 ("m2l_launches", ctypes.c_int32), ("m2l_mixed_launches", ctypes.c_int32),
                 ("t_leaf_kernel_ms", ctypes.c_double), ("t_m2l_mixed_ms", ctypes.c_double),
-                ("m2l_ops_mixed", ctypes.c_int64), ("num_boxes", ctypes.c_int64), ("xw_pairs", ctypes.c_int64)]
+                ("m2l_ops_mixed", ctypes.c_int64), ("num_boxes", ctypes.c_int64), ("xw_pairs", ctypes.c_int64),
+                ("terms_min", ctypes.c_int32), ("fallback", ctypes.c_int32), ("check_rel_l2", ctypes.c_double),
+                ("check_rel_l2_field", ctypes.c_double), ("m2l_flops", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -151,6 +155,26 @@ def lib():
         L.lfmm_tree3d_adaptive.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points3d), ctypes.c_int, ctypes.c_int] + \
             [ctypes.c_void_p] * 13
         L.lfmm_tree3d_adaptive.restype = ctypes.c_int
+        L.lfmm_fmm3d_orders.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points3d), ctypes.POINTER(Points3d),
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Report)]
+        L.lfmm_fmm3d_orders.restype = ctypes.c_int
+        L.lfmm_set_order_model.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int]
+        L.lfmm_set_order_model.restype = ctypes.c_int
+        L.lfmm_order_features.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points3d), ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.lfmm_order_features.restype = ctypes.c_int
+        L.lfmm_order_predict.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                         ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+        L.lfmm_order_predict.restype = ctypes.c_int
+        L.lfmm_order_loss_grad.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+        L.lfmm_order_loss_grad.restype = ctypes.c_int
+        L.lfmm_order_train.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64,
+                                       ctypes.c_void_p]
+        L.lfmm_order_train.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -280,13 +304,21 @@ def _host_charges(name, a, n, width=None):
 
 
 def _solve(dim, xyz, q, eps, terms, tree_depth, leaf_max, exclude_self, timings, count_ops, ctx, out, targets,
-           mixed, adaptive=False):
+           mixed, adaptive=False, learned_order=False, particle_terms=None):
     fn = lib().lfmm_fmm3d if dim == 3 else lib().lfmm_fmm2d
     name = "lfmm_fmm3d" if dim == 3 else "lfmm_fmm2d"
     flags = ((FLAG_EXCLUDE_SELF if exclude_self else 0) | (FLAG_TIMINGS if timings else 0) |
              (FLAG_COUNT_OPS if count_ops else 0) | (FLAG_MIXED if mixed else 0) |
-             (FLAG_ADAPTIVE if adaptive else 0))
+             (FLAG_ADAPTIVE if adaptive else 0) | (FLAG_LEARNED_ORDER if learned_order else 0))
     rep = Report()
+    req_ptr = None
+    if particle_terms is not None:
+        if dim != 3:
+            raise ValueError("per-particle orders are a 3D option")
+        def fn(c, pts, tg, eps_, terms_, depth_, leaf_, mem, flags_, pot_, field_, rep_):
+            return lib().lfmm_fmm3d_orders(c, pts, tg, ctypes.c_void_p(req_ptr), depth_, leaf_, mem, flags_, pot_,
+                                           field_, rep_)
+        name = "lfmm_fmm3d_orders"
     if _is_torch_cuda(xyz):
         import torch
         n = _dev_points("positions", xyz, dim, xyz.device)
@@ -304,6 +336,13 @@ def _solve(dim, xyz, q, eps, terms, tree_depth, leaf_max, exclude_self, timings,
             pot, field = out
             _dev_out("out potential", pot, m, xyz.device)
             _dev_out("out field", field, m, xyz.device, dim)
+        if particle_terms is not None:
+            if not _is_torch_cuda(particle_terms) or particle_terms.dtype != torch.int32:
+                raise TypeError("particle_terms must be a CUDA int32 tensor like the other device inputs")
+            if particle_terms.device != xyz.device or tuple(particle_terms.shape) != (m,) or \
+                    (m > 0 and not particle_terms.is_contiguous()):
+                raise ValueError("particle_terms must be contiguous with shape (%d,) on %s" % (m, xyz.device))
+            req_ptr = particle_terms.data_ptr()
         ctx = ctx or default_context()
         _bind_stream(ctx, xyz)
         rc = fn(ctx.handle, ctypes.byref(pts), tg, float(eps), int(terms), int(tree_depth), int(leaf_max), MEM_DEVICE,
@@ -326,6 +365,13 @@ def _solve(dim, xyz, q, eps, terms, tree_depth, leaf_max, exclude_self, timings,
         tg = ctypes.byref(Points3d(m, targets.ctypes.data, dim, None, 1))
     pot = np.empty(m)
     field = np.empty((m, dim))
+    if particle_terms is not None:
+        if _is_torch_cuda(particle_terms):
+            raise TypeError("particle_terms is a CUDA tensor but the sources are host arrays")
+        pt_ = np.ascontiguousarray(particle_terms, dtype=np.int32)
+        if pt_.shape != (m,):
+            raise ValueError("particle_terms must have shape (%d,), got %s" % (m, pt_.shape))
+        req_ptr = pt_.ctypes.data
     ctx = ctx or default_context()
     rc = fn(ctx.handle, ctypes.byref(pts), tg, float(eps), int(terms), int(tree_depth), int(leaf_max), MEM_HOST, flags,
             pot.ctypes.data, field.ctypes.data, ctypes.byref(rep))
@@ -334,7 +380,8 @@ def _solve(dim, xyz, q, eps, terms, tree_depth, leaf_max, exclude_self, timings,
 
 
 def fmm3d(xyz, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exclude_self=False,
-          timings=False, count_ops=False, ctx=None, out=None, targets=None, mixed=False, adaptive=False):
+          timings=False, count_ops=False, ctx=None, out=None, targets=None, mixed=False, adaptive=False,
+          learned_order=False, particle_terms=None):
     """3D FMM: potential (m,) and field (m,3) at every target.
 
     xyz (n,3) and q (n,) are the sources; targets (m,3) defaults to the
@@ -345,9 +392,14 @@ def fmm3d(xyz, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exc
     with FP32 accumulation (LFMM_FLAG_MIXED); everything else stays FP64.
     adaptive=True builds the adaptive octree (LFMM_FLAG_ADAPTIVE): boxes of
     more than leaf_max particles (64 if 0) are split, leaves on any level.
+    learned_order=True lets the context's order network (set_order_model)
+    pick the order of every leaf (LFMM_FLAG_LEARNED_ORDER; terms and eps are
+    ignored; a sampled direct sum checks the run, report['fallback'] = 1 when
+    it was rerun at p = 16).  particle_terms (int32, one per target) asks for
+    at least that many terms at each target (lfmm_fmm3d_orders).
     """
     return _solve(3, xyz, q, eps, terms, tree_depth, leaf_max, exclude_self, timings, count_ops, ctx, out, targets,
-                  mixed, adaptive)
+                  mixed, adaptive, learned_order, particle_terms)
 
 
 def fmm2d(xy, q, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exclude_self=False,
@@ -579,6 +631,80 @@ def _tree(dim, xyz, q, tree_depth, leaf_max, ctx):
     return out
 
 
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and a.shape != shape:
+        raise ValueError("expected shape %s, got %s" % (shape, a.shape))
+    return a
+
+
+def set_order_model(model, confidence=0.95, check_tol=1e-6, check_samples=256, ctx=None):
+    """Installs the order network (ORDER_MODEL_SIZE floats) on the context for
+    learned_order solves; None removes it (lfmm_set_order_model)."""
+    ctx = ctx or default_context()
+    if model is None:
+        _check(lib().lfmm_set_order_model(ctx.handle, None, 0.0, 0.0, 0), "lfmm_set_order_model")
+        return
+    m = _f32(model, (ORDER_MODEL_SIZE,))
+    _check(lib().lfmm_set_order_model(ctx.handle, m.ctypes.data, float(confidence), float(check_tol),
+                                      int(check_samples)), "lfmm_set_order_model")
+
+
+def order_features(xyz, q, leaf_max=64, tree_depth=FMM_AUTO, ctx=None):
+    """Per-leaf features (nleaf, 8) of the adaptive octree of a solve with the
+    same leaf_max and tree_depth, and the leaf of every particle (n,)."""
+    xyz = _host_points("positions", xyz, 3)
+    q = _host_charges("charges", q, len(xyz))
+    ctx = ctx or default_context()
+    pts = Points3d(len(xyz), xyz.ctypes.data, 3, q.ctypes.data, 1)
+    nl = np.zeros(1, np.int64)
+    _check(lib().lfmm_order_features(ctx.handle, ctypes.byref(pts), int(tree_depth), int(leaf_max), nl.ctypes.data,
+                                     None, None), "lfmm_order_features")
+    feat = np.zeros((int(nl[0]), ORDER_FEATURES), np.float32)
+    leaf = np.zeros(len(xyz), np.int32)
+    _check(lib().lfmm_order_features(ctx.handle, ctypes.byref(pts), int(tree_depth), int(leaf_max), nl.ctypes.data,
+                                     feat.ctypes.data, leaf.ctypes.data), "lfmm_order_features")
+    return feat, leaf
+
+
+def order_predict(model, features, confidence=0.95, ctx=None):
+    """Class probabilities (n, 13) and chosen terms (n,) of feature rows."""
+    ctx = ctx or default_context()
+    m = _f32(model, (ORDER_MODEL_SIZE,))
+    f = _f32(features)
+    n = len(f)
+    probs = np.zeros((n, ORDER_CLASSES), np.float32)
+    terms = np.zeros(n, np.int32)
+    _check(lib().lfmm_order_predict(ctx.handle, m.ctypes.data, f.ctypes.data, n, float(confidence), probs.ctypes.data,
+                                    terms.ctypes.data), "lfmm_order_predict")
+    return probs, terms
+
+
+def order_loss_grad(model, features, labels, ctx=None):
+    """Mean cross-entropy of labelled feature rows and its gradient (GPU)."""
+    ctx = ctx or default_context()
+    m = _f32(model, (ORDER_MODEL_SIZE,))
+    f = _f32(features)
+    y = np.ascontiguousarray(labels, dtype=np.int32)
+    loss = np.zeros(1, np.float32)
+    grad = np.zeros(ORDER_MODEL_SIZE, np.float32)
+    _check(lib().lfmm_order_loss_grad(ctx.handle, m.ctypes.data, f.ctypes.data, y.ctypes.data, len(y), loss.ctypes.data,
+                                      grad.ctypes.data), "lfmm_order_loss_grad")
+    return float(loss[0]), grad
+
+
+def order_train(model, features, labels, epochs=30, batch=1024, lr=3e-3, seed=1, ctx=None):
+    """Adam on the GPU; returns (trained model, mean loss per epoch)."""
+    ctx = ctx or default_context()
+    m = _f32(model, (ORDER_MODEL_SIZE,)).copy()
+    f = _f32(features)
+    y = np.ascontiguousarray(labels, dtype=np.int32)
+    hist = np.zeros(int(epochs), np.float32)
+    _check(lib().lfmm_order_train(ctx.handle, m.ctypes.data, f.ctypes.data, y.ctypes.data, len(y), int(epochs),
+                                  int(batch), float(lr), int(seed), hist.ctypes.data), "lfmm_order_train")
+    return m, hist
+
+
 def tree3d_adaptive(xyz, leaf_max=64, tree_depth=FMM_AUTO, ctx=None):
     """The adaptive octree and its U, X, W lists (lfmm_tree3d_adaptive) as a
     dict of numpy arrays: depth, geom, box_level, box_key, box_range (sorted
@@ -684,3 +810,6 @@ class FastMultipole3d:
             xyz, q = p[:, :3], p[:, 3]
         self.potentials, self.forces, self.report = fmm3d(xyz, q, eps=eps, terms=N, tree_depth=tree_depth,
                                                           leaf_max=leaf_max, ctx=ctx)
+
+
+from . import order  # noqa: E402  (the order network's training data and driver)

@@ -27,6 +27,11 @@ using namespace lfmm;
 
 // First M2L level on the matrix cores: the two coarsest levels stay FP64.
 constexpr int kMixedFirstLevel = 4;
+// Adaptive octree, fixed order: the M2L of a level whose parent level holds
+// at least this many boxes runs over the parents with children only (a
+// parent list), so no lane holds a leaf parent; smaller levels keep the
+// direction-split launch over all parents.
+constexpr int64_t kBucketMinParents = 4096;
 
 int get_tables(lfmm_ctx* ctx, int N, const DevTables** out)
 {
@@ -344,14 +349,18 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
 
 int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, double eps, int terms_in,
                 int depth_in, int leaf_max, int memory, unsigned flags, double* pot_out, double* field_out,
-                lfmm_report* rep, const DevTables* tables)
+                lfmm_report* rep, const DevTables* tables, const int32_t* req)
 {
     const auto T0 = std::chrono::steady_clock::now();
     const bool timed = (flags & LFMM_FLAG_TIMINGS) != 0;
     const int64_t n_out = Tg ? Tg->n : S->n;
-    const int N = terms_3d(eps, terms_in);
-    if (N < 1) return LFMM_ERR_TERMS;
-    const int NX = tri(N);
+    // per-box orders: chosen by the order network, or requested per particle
+    const bool learned = (flags & LFMM_FLAG_LEARNED_ORDER) != 0 && !req;
+    const bool per_box = learned || req;
+    if (learned && !ctx->order_model) return LFMM_ERR_INVALID;
+    if (per_box && tables) return LFMM_ERR_UNSUPPORTED;
+    int N = terms_3d(eps, terms_in);
+    if (!per_box && N < 1) return LFMM_ERR_TERMS;
     hipStream_t st = ctx->stream;
     Arena& A = ctx->arena;
     A.reset();
@@ -360,7 +369,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     if (rc != LFMM_OK) return rc;
     const int64_t n = DP.n;
     Tree T;
-    const bool adaptive = (flags & LFMM_FLAG_ADAPTIVE) != 0;
+    const bool adaptive = per_box || (flags & LFMM_FLAG_ADAPTIVE) != 0;
     // the adaptive octree splits boxes of more than leaf_max particles (64
     // when not given), down to the depth at which every box holds that many
     if (adaptive && leaf_max <= 0) leaf_max = 64;
@@ -383,6 +392,54 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     std::vector<int32_t*>& lut = T.lut;
     const uint32_t max_leaf = T.max_leaf;
     const int64_t nleaf = adaptive ? T.nleaf : cnt[leaf_level];
+
+    // ---- per-box orders: leaf terms, box terms (a parent: the largest of its
+    // children) and the (level, terms) launch buckets of levels >= 4
+    BoxOrders O;
+    if (adaptive && !per_box && N >= 1 && N <= LFMM_MAX_TERMS_3D) {
+        // one order: the buckets are the parent lists of the levels
+        uint32_t* leaf_terms = take<uint32_t>(A, T.nleaf);
+        if (!leaf_terms) return LFMM_ERR_NOMEM;
+        LFMM_HIP(fill_u32(leaf_terms, T.nleaf, (uint32_t)N, st));
+        if ((rc = order_box_terms(ctx, T, leaf_terms, kMixedFirstLevel, O)) != LFMM_OK) return rc;
+    }
+    if (per_box) {
+        const DevTables* DTf = nullptr;  // direction tables (the same for every N)
+        rc = get_tables(ctx, kOrderMinTerms + kOrderClasses - 1, &DTf);
+        if (rc != LFMM_OK) return rc;
+        uint32_t* leaf_terms = take<uint32_t>(A, T.nleaf);
+        if (!leaf_terms) return LFMM_ERR_NOMEM;
+        if (learned) {
+            float* feat = take<float>(A, T.nleaf * kOrderFeatures);
+            if (!feat) return LFMM_ERR_NOMEM;
+            if ((rc = order_features(ctx, T, DTf, feat)) != LFMM_OK) return rc;
+            rc = order_predict(st, ctx->order_model, feat, T.nleaf, (float)ctx->order_conf, nullptr, (int32_t*)leaf_terms);
+            if (rc != LFMM_OK) return rc;
+        } else {
+            const int32_t* dreq = req;
+            if (memory == LFMM_MEM_HOST) {
+                int32_t* r = take<int32_t>(A, n_out);
+                if (!r) return LFMM_ERR_NOMEM;
+                LFMM_HIP(hipMemcpyAsync(r, req, sizeof(int32_t) * n_out, hipMemcpyHostToDevice, st));
+                dreq = r;
+            }
+            rc = order_leaf_terms_from_requests(st, T, dreq, n_out, 1, LFMM_MAX_TERMS_3D, leaf_terms);
+            if (rc != LFMM_OK) return rc;
+        }
+        if ((rc = order_box_terms(ctx, T, leaf_terms, kMixedFirstLevel, O)) != LFMM_OK) return rc;
+        N = O.max_terms;
+        if (N < 1 || N > LFMM_MAX_TERMS_3D) return LFMM_ERR_TERMS;
+        // the operator tables of every bucket's order, before any launch
+        for (int l = kMixedFirstLevel; l < depth; ++l)
+            for (int t = 1; t <= LFMM_MAX_TERMS_3D; ++t) {
+                const int key = (l - 1) * 32 + t;
+                if (O.seg[key + 1] == O.seg[key]) continue;
+                const DevTables* DB = nullptr;
+                if ((rc = get_tables(ctx, t, &DB)) != LFMM_OK) return rc;
+                if ((flags & LFMM_FLAG_MIXED) && (rc = get_m16(ctx, t, &DB)) != LFMM_OK) return rc;
+            }
+    }
+    const int NX = tri(N);
     const double t_tree = timed ? ms_since(T0) : 0.0;
 
     // ---- expansions
@@ -481,7 +538,61 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             L.parent_local = (l > 2) ? inner[l - 1] : nullptr;
             L.dst = inner[l];
             L.tiles = (int)((cnt[l - 1] + rtr_tile(N) - 1) / rtr_tile(N));
-            if (mixed && l >= kMixedFirstLevel) {
+            const bool bucketed = adaptive && l >= kMixedFirstLevel && (per_box || cnt[l - 1] >= kBucketMinParents);
+            if (bucketed) {
+                // one launch per order among the level's parents with children (parent lists)
+                for (int t = 1; t <= LFMM_MAX_TERMS_3D; ++t) {
+                    const int key = (l - 1) * 32 + t;
+                    const int64_t b0 = O.seg[key], b1 = O.seg[key + 1];
+                    if (b1 == b0) continue;
+                    const DevTables* DB = &ctx->tables.at(t);
+                    RtrLevel Lb = L;
+                    Lb.plist = O.plist + b0;
+                    Lb.num_parents = (int)(b1 - b0);
+                    Lb.stride = NX;
+                    Lb.dsplit = 1;
+                    Lb.tiles = (int)((b1 - b0 + rtr_tile(t) - 1) / rtr_tile(t));
+                    if (mixed) {
+                        M2L16Args m{};
+                        m.kmat = DB->m16;
+                        m.kpre = DB->m16_kpre;
+                        m.outer = outer[l];
+                        m.xp = xp;
+                        m.xmax = xmax + l;
+                        m.class_dirs = DB->rt.class_dirs;
+                        m.dir_vec = DB->rt.dir_vec;
+                        m.level = l;
+                        m.num_parents = (int)cnt[l - 1];
+                        m.child_begin = L.child_begin;
+                        m.child_keys = L.child_keys;
+                        m.lut = L.lut;
+                        m.keys = L.keys;
+                        m.plut = lut[l - 1];
+                        m.pkeys = lkeys[l - 1];
+                        m.size = L.size;
+                        m.kinv = DB->m16_kinv;
+                        m.dst = inner[l];
+                        m.plist = Lb.plist;
+                        m.plist_count = Lb.num_parents;
+                        m.stride = NX;
+                        m.tiles = (int)((b1 - b0 + m16_targets(t) - 1) / m16_targets(t));
+                        m.dsplit = 1;
+                        LFMM_HIP(launch_m2l16_pack(t, m, st));
+                        LFMM_HIP(launch_rtr(t, 2, DB->rt, Lb, st));
+                        if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+                        LFMM_HIP(launch_m2l16(t, m, st));
+                        if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+                        ev_mixed.push_back(1);
+                        ++mixed_launches;
+                    } else {
+                        if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+                        LFMM_HIP(launch_rtr(t, 1, DB->rt, Lb, st));
+                        if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+                        ev_mixed.push_back(0);
+                    }
+                    ++m2l_launches;
+                }
+            } else if (mixed && l >= kMixedFirstLevel) {
                 // FP64 L2L into the level, then += the matrix-core M2L
                 M2L16Args m{};
                 m.kmat = DT->m16;
@@ -533,7 +644,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                 if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
                 ev_mixed.push_back(0);
             }
-            ++m2l_launches;
+            if (!bucketed) ++m2l_launches;
             if (adaptive && T.xseg[l + 1] > T.xseg[l]) {
                 // X lists of the level: source particles into the locals (P2L)
                 P2MArgs px{src, nullptr, T.keys_all, (int)(T.xseg[l + 1] - T.xseg[l]), l, geom, DT->knm, inner_all};
@@ -558,6 +669,10 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         dpot = pot_out ? take<double>(A, n_out) : nullptr;
         dfield = field_out ? take<double>(A, 3 * n_out) : nullptr;
         if ((pot_out && !dpot) || (field_out && !dfield)) return LFMM_ERR_NOMEM;
+    }
+    if (learned) {  // the sampled check reads both outputs
+        if (!dpot && !(dpot = take<double>(A, n_out))) return LFMM_ERR_NOMEM;
+        if (!dfield && !(dfield = take<double>(A, 3 * n_out))) return LFMM_ERR_NOMEM;
     }
     LeafArgs la{};
     la.src = src;
@@ -608,7 +723,15 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     if (far) LFMM_HIP(launch_l2p(N, la, st));
     LFMM_HIP(launch_p2p(la, p2p_ws, p2p_scan, st));  // near field, several targets per lane
     if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
-    if (memory == LFMM_MEM_HOST) {
+    // learned orders: the sampled direct-sum check judges the run
+    double chk_p = 0.0, chk_f = 0.0;
+    if (learned) {
+        rc = order_check(ctx, DP, src, n, n_out, DP.q != nullptr, dpot, dfield, !(flags & LFMM_FLAG_EXCLUDE_SELF),
+                         ctx->order_samples, &chk_p, &chk_f);
+        if (rc != LFMM_OK) return rc;
+    }
+    const bool fail_check = learned && !(chk_p <= ctx->order_tol);
+    if (memory == LFMM_MEM_HOST && !fail_check) {
         if (pot_out) LFMM_HIP(hipMemcpyAsync(pot_out, dpot, sizeof(double) * n_out, hipMemcpyDeviceToHost, st));
         if (field_out) LFMM_HIP(hipMemcpyAsync(field_out, dfield, sizeof(double) * 3 * n_out, hipMemcpyDeviceToHost, st));
     }
@@ -627,6 +750,11 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         rep->p2p_pairs = -1;
         rep->m2l_launches = m2l_launches;
         rep->m2l_mixed_launches = mixed_launches;
+        rep->terms_min = per_box ? O.min_leaf_terms : N;
+        rep->fallback = 0;
+        rep->check_rel_l2 = chk_p;
+        rep->check_rel_l2_field = chk_f;
+        rep->m2l_flops = -1.0;
         if (timed) {
             double tm = 0.0, tx = 0.0;
             for (size_t i = 0; i + 1 < ev_leaf; i += 2) {
@@ -658,9 +786,14 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             rep->m2l_ops_mixed = 0;
             if (mixed)
                 for (int l = kMixedFirstLevel; l < depth; ++l) rep->m2l_ops_mixed += per[l];
+            rep->m2l_flops = (double)rep->m2l_ops * (double)m2l_flops_per_op(N);
+            if (per_box) {
+                rc = order_m2l_flops(ctx, T, O, N, kMixedFirstLevel, &rep->m2l_flops);
+                if (rc != LFMM_OK) return rc;
+            }
         }
     }
-    return LFMM_OK;
+    return fail_check ? kNeedFallback : LFMM_OK;
 }
 
 bool valid_points(const lfmm_points3d* p, bool need_q, int dim)
@@ -946,8 +1079,23 @@ int lfmm_fmm3d(lfmm_ctx* c, const lfmm_points3d* sources, const lfmm_points3d* t
     std::lock_guard<std::mutex> g(c->mu);
     if (hipSetDevice(c->device) != hipSuccess) return LFMM_ERR_HIP;
     try {
-        return fmm3d_solve(c, sources, tg, eps, terms, tree_depth, leaf_max, memory, flags, potential, field, report,
-                           nullptr);
+        int rc = fmm3d_solve(c, sources, tg, eps, terms, tree_depth, leaf_max, memory, flags, potential, field, report,
+                             nullptr);
+        if (rc == kNeedFallback) {
+            // the learned orders missed the sampled check: the fixed order
+            // (p = 16) on the same adaptive tree
+            lfmm_report keep{};
+            if (report) keep = *report;
+            rc = fmm3d_solve(c, sources, tg, eps, kOrderMinTerms + kOrderClasses - 1, tree_depth, leaf_max, memory,
+                             (flags & ~LFMM_FLAG_LEARNED_ORDER) | LFMM_FLAG_ADAPTIVE, potential, field, report, nullptr);
+            if (report) {
+                report->fallback = 1;
+                report->check_rel_l2 = keep.check_rel_l2;
+                report->check_rel_l2_field = keep.check_rel_l2_field;
+                report->terms_min = keep.terms_min;
+            }
+        }
+        return rc;
     } catch (const std::bad_alloc&) {
         return LFMM_ERR_NOMEM;
     } catch (...) {

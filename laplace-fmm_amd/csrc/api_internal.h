@@ -6,6 +6,7 @@
 #include "ops3d.h"
 #include "tree.h"
 #include "adaptive.h"
+#include "order.h"
 
 #include <hip/hip_runtime.h>
 
@@ -101,6 +102,10 @@ struct lfmm_ctx {
     hipEvent_t switch_event = nullptr;  // orders a stream switch (lfmm_set_stream)
     void* vec_tmp = nullptr;            // two field buffers of the vector-charge device path
     size_t vec_tmp_bytes = 0;
+    float* order_model = nullptr;       // the order network (lfmm_set_order_model), on the device
+    double order_conf = 0.95;           // cumulative class probability of the chosen order
+    double order_tol = 1e-6;            // sampled rel-L2 above which the fixed order reruns the solve
+    int order_samples = 256;            // targets of the sampled direct-sum check
     std::mutex mu;
 };
 
@@ -188,15 +193,55 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
 // Turns the uniform tree T (built with keep_parents) into the adaptive octree
 // of leaf size leaf_max and builds its U, X and W lists (adaptive.hip).
 int build_adaptive(lfmm_ctx* ctx, int leaf_max, Tree& T);
+// ---- per-box orders (order.hip)
+struct BoxOrders {
+    uint32_t* box_terms = nullptr;   // [total] terms of each box (a parent: the largest of its children)
+    int32_t* plist = nullptr;        // parents with children sorted by (level, terms), indices at their level
+    std::vector<int64_t> seg;        // plist entries of key level*32 + terms: [seg[key], seg[key+1])
+    int max_terms = 0;               // over the leaves
+    int min_leaf_terms = 0;
+};
+// 8 features per leaf of an adaptive tree (feat: nleaf * 8 floats, device).
+int order_features(lfmm_ctx* ctx, const Tree& T, const DevTables* DT, float* feat);
+// Order network forward: probabilities (n x 13) and/or terms by the confidence rule.
+int order_predict(hipStream_t st, const float* model, const float* feat, int64_t n, float conf, float* probs,
+                  int32_t* terms);
+// Cross-entropy gradient over samples idx[0..n) (or 0..n), scaled by
+// inv_batch, and the summed loss; workspace gpart [order_grad_parts(n) *
+// LFMM_ORDER_MODEL_SIZE], lpart [order_grad_parts(n)].
+int64_t order_grad_parts(int64_t n);
+int order_grad(hipStream_t st, const float* model, const float* feat, const int32_t* label, const int32_t* idx,
+               int64_t n, float inv_batch, float* grad, float* loss, float* gpart, float* lpart);
+int order_accum(hipStream_t st, const float* loss, float* acc);
+int order_adam(hipStream_t st, float* model, const float* grad, float* m, float* v, float lr, int step);
+// Leaf terms = the largest request of the leaf's particles (requests in input order), clamped to [lo, hi].
+int order_leaf_terms_from_requests(hipStream_t st, const Tree& T, const int32_t* req, int64_t n_req, int lo, int hi,
+                                   uint32_t* leaf_terms);
+// Box terms and the (level, terms) launch buckets of the parents of levels >= first_bucket_level.  Synchronises.
+int order_box_terms(lfmm_ctx* ctx, const Tree& T, const uint32_t* leaf_terms, int first_bucket_level, BoxOrders& O);
+// Algorithmic FP64 flops of all M2L operations at their orders (levels below
+// first_bucket_level at N).  Synchronises.
+int order_m2l_flops(lfmm_ctx* ctx, const Tree& T, const BoxOrders& O, int N, int first_bucket_level, double* flops);
+// Sampled direct-sum check: rel-L2 of potential (self pair excluded) and field on `samples` targets.
+int order_check(lfmm_ctx* ctx, const DevPoints& DP, const double4* src, int64_t n, int64_t n_out, bool has_q,
+                const double* pot, const double* field, bool fmm_has_self, int samples, double* rel_pot,
+                double* rel_field);
+
 // Near-field pair count of an adaptive tree's U lists (ws: 1 counter).  Synchronises.
 int ad_count_pairs(const int32_t* leaf_begin, const uint32_t* u_off, const int2* u_rng, int64_t nleaf,
                    unsigned long long* ws, int64_t* pairs, hipStream_t st);
 // The 3D solve behind lfmm_fmm3d (arguments checked, ctx->mu held, device
 // set).  tables: operator tables already on the device, or null to take the
 // context's cached ones (built on first use).
+// req: per-box orders requested per particle (input order, memory as the
+// outputs; n_out entries), or null.
 int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, double eps, int terms_in,
                 int depth_in, int leaf_max, int memory, unsigned flags, double* pot_out, double* field_out,
-                lfmm_report* rep, const DevTables* tables);
+                lfmm_report* rep, const DevTables* tables, const int32_t* req = nullptr);
+// fmm3d_solve's status when a learned-order solve failed its sampled check
+// (the caller reruns it with the fixed order)
+constexpr int kNeedFallback = -2;
+int get_tables(lfmm_ctx* ctx, int N, const DevTables** out);
 int terms_3d(double eps, int terms);
 constexpr int kNeedSolve = -1;
 int fmm3d_prologue(lfmm_ctx* c, const lfmm_points3d* sources, const lfmm_points3d* targets, double eps, int terms,

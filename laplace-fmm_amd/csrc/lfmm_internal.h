@@ -34,6 +34,17 @@ constexpr int rot_size(int N) { return rot_base(N); }
 constexpr int tr_base(int N, int k) { return rot_base(N) - rot_base(N - k); }
 constexpr int tr_size(int N) { return rot_base(N); }
 
+// Algorithmic FP64 flops of one rotation-based M2L with N terms: Rz and Ry
+// forward and back, 2 (6 tri(N) + sum_{n<N} (n+1)(4n+2)), plus the axial
+// translation, 4 sum_{t=1..N} t^2.
+constexpr int64_t m2l_flops_per_op(int N)
+{
+    int64_t rot = 0, tr = 0;
+    for (int n = 0; n < N; ++n) rot += (int64_t)(n + 1) * (4 * n + 2);
+    for (int t = 1; t <= N; ++t) tr += (int64_t)t * t;
+    return 2 * (6 * (int64_t)tri(N) + rot) + 4 * tr;
+}
+
 // Host-built FP64 operator tables for one expansion size N.
 struct OpTables {
     int N = 0;

@@ -882,14 +882,21 @@ static int auto_terms_3d(double eps, int N)
 	return N < ORC_MAX_TERMS_3D ? N : ORC_MAX_TERMS_3D;
 }
 
-int orc_fmm3d(int64_t n_src, const double* src_xyz, const double* src_q,
-              int64_t n_tgt, const double* tgt_xyz,
-              double eps, int N_, int depth_, double* pot_out, double* field_out, double* timings)
+/* The 3D solve.  pterms (NULL: one order N for every box) gives per-box
+ * truncation orders, a restatement of the GPU's per-box orders (no
+ * reference counterpart): every leaf takes the largest request of its
+ * particles (pterms in input order, one per output), a box with children the
+ * largest of its children, and the M2L and L2L into a box of level >= 4 run
+ * with the order of its parent (levels 2-3 and P2M, M2M, L2P with the
+ * largest order; coefficients beyond a box's order stay zero). */
+static int fmm3d_impl(int64_t n_src, const double* src_xyz, const double* src_q,
+                      int64_t n_tgt, const double* tgt_xyz,
+                      double eps, int N_, int depth_, double* pot_out, double* field_out, double* timings,
+                      const int32_t* pterms)
 {
 	orc_init();
 	double T0 = omp_get_wtime(), t;
-	const int N = auto_terms_3d(eps, N_);
-	const int Nx2 = tri(N);
+	int N = auto_terms_3d(eps, N_);
 	const int64_t n = n_src + n_tgt;
 	const int64_t out_n = n_tgt ? n_tgt : n_src;
 	int depth = depth_;
@@ -908,6 +915,32 @@ int orc_fmm3d(int64_t n_src, const double* src_xyz, const double* src_q,
 	free(pos); free(q);
 	if (!tr) return -1;
 	double t_tree = omp_get_wtime() - T0;
+	/* per-box orders: bt[level][box] */
+	int** bt = NULL;
+	if (pterms) {
+		bt = (int**)calloc(depth, sizeof(int*));
+		for (int l = 0; l < depth; ++l) bt[l] = (int*)calloc(tr->lv[l].size ? tr->lv[l].size : 1, sizeof(int));
+		orc_level* LL = &tr->lv[depth - 1];
+		int64_t* leaf_of = (int64_t*)malloc(sizeof(int64_t) * n);
+		for (int64_t li = 0; li < LL->size; ++li) {
+			for (int64_t s = LL->sb[li]; s < LL->se[li]; ++s) leaf_of[s] = li;
+			for (int64_t s = LL->tb[li]; s < LL->te[li]; ++s) leaf_of[s] = li;
+		}
+		N = 1;
+		for (int64_t i = 0; i < out_n; ++i) {
+			int v = pterms[i] < 1 ? 1 : (pterms[i] > ORC_MAX_TERMS_3D ? ORC_MAX_TERMS_3D : pterms[i]);
+			int64_t li = leaf_of[tr->pos_map[i]];
+			if (v > bt[depth - 1][li]) bt[depth - 1][li] = v;
+			if (v > N) N = v;
+		}
+		free(leaf_of);
+		for (int l = depth - 2; l >= 0; --l)
+			for (int64_t j = 0; j < tr->lv[l].size; ++j)
+				for (int64_t k = tr->lv[l].sb[j]; k < tr->lv[l].se[j]; ++k)
+					if (bt[l + 1][k] > bt[l][j]) bt[l][j] = bt[l + 1][k];
+	}
+	const int Nx2 = tri(N);
+#define BOX_TERMS(i, j) ((bt && (i) >= 4) ? bt[(i) - 1][tr->lv[i].parent[j]] : N)
 
 	cplx** outer = (cplx**)calloc(depth, sizeof(cplx*));
 	cplx** inner = (cplx**)calloc(depth, sizeof(cplx*));
@@ -971,7 +1004,7 @@ int orc_fmm3d(int64_t n_src, const double* src_xyz, const double* src_q,
 					int64_t s = L->far_idx[a];
 					double p0[3] = {L->centre[3 * s] - L->centre[3 * j], L->centre[3 * s + 1] - L->centre[3 * j + 1],
 					                L->centre[3 * s + 2] - L->centre[3 * j + 2]};
-					err |= rtr_op(OP_M2L, N, outer[i] + s * Nx2, p0, inner[i] + Nx2 * j, w1, w2);
+					err |= rtr_op(OP_M2L, BOX_TERMS(i, j), outer[i] + s * Nx2, p0, inner[i] + Nx2 * j, w1, w2);
 				}
 		}
 	}
@@ -986,7 +1019,7 @@ int orc_fmm3d(int64_t n_src, const double* src_xyz, const double* src_q,
 				int64_t p = L->parent[j];
 				double p0[3] = {T->centre[3 * p] - L->centre[3 * j], T->centre[3 * p + 1] - L->centre[3 * j + 1],
 				                T->centre[3 * p + 2] - L->centre[3 * j + 2]};
-				err |= rtr_op(OP_L2L, N, inner[i - 1] + Nx2 * p, p0, inner[i] + Nx2 * j, w1, w2);
+				err |= rtr_op(OP_L2L, BOX_TERMS(i, j), inner[i - 1] + Nx2 * p, p0, inner[i] + Nx2 * j, w1, w2);
 			}
 		}
 	}
@@ -1071,12 +1104,30 @@ int orc_fmm3d(int64_t n_src, const double* src_xyz, const double* src_q,
 	free(bufs);
 	for (int l = 0; l < depth; ++l) { free(outer[l]); free(inner[l]); }
 	free(outer); free(inner);
+	if (bt) {
+		for (int l = 0; l < depth; ++l) free(bt[l]);
+		free(bt);
+	}
+#undef BOX_TERMS
 	orc_tree3d_free(tr);
 	if (timings) {
 		timings[0] = t_tree; timings[1] = t_p2m; timings[2] = t_m2m; timings[3] = t_down; timings[4] = t_leaf;
 		timings[5] = omp_get_wtime() - T0;
 	}
 	return err ? -2 : 0;
+}
+
+int orc_fmm3d(int64_t n_src, const double* src_xyz, const double* src_q,
+              int64_t n_tgt, const double* tgt_xyz,
+              double eps, int N_, int depth_, double* pot_out, double* field_out, double* timings)
+{
+	return fmm3d_impl(n_src, src_xyz, src_q, n_tgt, tgt_xyz, eps, N_, depth_, pot_out, field_out, timings, NULL);
+}
+
+int orc_fmm3d_orders(int64_t n_src, const double* src_xyz, const double* src_q, int64_t n_tgt, const double* tgt_xyz,
+                     const int32_t* pterms, int depth, double* pot_out, double* field_out)
+{
+	return fmm3d_impl(n_src, src_xyz, src_q, n_tgt, tgt_xyz, 1e-8, 1, depth, pot_out, field_out, NULL, pterms);
 }
 
 /* ===================================================================== */

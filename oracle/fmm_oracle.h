@@ -79,6 +79,10 @@ void orc_tree3d_lists(const orc_tree3d* t, int level, int far, int64_t* offsets,
 int orc_fmm3d(int64_t n_src, const double* src_xyz, const double* src_q,
               int64_t n_tgt, const double* tgt_xyz,
               double eps, int N, int depth, double* pot, double* field, double* timings);
+/* The same with per-box truncation orders requested per output particle
+ * (pterms, input order; see fmm3d_impl in fmm_oracle.c). */
+int orc_fmm3d_orders(int64_t n_src, const double* src_xyz, const double* src_q, int64_t n_tgt, const double* tgt_xyz,
+                     const int32_t* pterms, int depth, double* pot_out, double* field_out);
 int orc_fmm2d(int64_t n_src, const double* src_xy, const double* src_q,
               int64_t n_tgt, const double* tgt_xy,
               double eps, int N, int depth, double* pot, double* field, double* timings);

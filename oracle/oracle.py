@@ -52,6 +52,9 @@ def lib():
             fn.restype = ctypes.c_int
             fn.argtypes = [ctypes.c_int64, _d, _d, ctypes.c_int64, _d, ctypes.c_double,
                            ctypes.c_int, ctypes.c_int, _d, _d, _d]
+        L.orc_fmm3d_orders.restype = ctypes.c_int
+        L.orc_fmm3d_orders.argtypes = [ctypes.c_int64, _d, _d, ctypes.c_int64, _d, ctypes.c_void_p, ctypes.c_int,
+                                       _d, _d]
         for f in ("orc_direct3d", "orc_direct2d"):
             fn = getattr(L, f)
             fn.argtypes = [ctypes.c_int64, _d, _d, ctypes.c_int64, _i64, _d, ctypes.c_int, _d, _d]
@@ -230,6 +233,23 @@ def fmm3d(src_xyz, src_q, N, depth, tgt_xyz=None, eps=1e-8, timings=False):
     if rc != 0:
         raise RuntimeError("oracle fmm3d failed: %d" % rc)
     return (pot, field, tm) if timings else (pot, field)
+
+
+def fmm3d_orders(src_xyz, src_q, pterms, depth):
+    """The 3D solve with per-box truncation orders requested per particle
+    (fmm3d_impl in fmm_oracle.c)."""
+    src_xyz = _f64(src_xyz).reshape(-1, 3)
+    src_q = _f64(src_q)
+    pt = np.ascontiguousarray(pterms, dtype=np.int32)
+    if pt.shape != (len(src_q),):
+        raise ValueError("one order request per particle")
+    pot = np.zeros(len(src_q))
+    field = np.zeros((len(src_q), 3))
+    rc = lib().orc_fmm3d_orders(len(src_q), _p(src_xyz), _p(src_q), 0, None, pt.ctypes.data, int(depth), _p(pot),
+                                _p(field))
+    if rc != 0:
+        raise RuntimeError("oracle fmm3d_orders failed: %d" % rc)
+    return pot, field
 
 
 class Tree2d(Tree3d):
