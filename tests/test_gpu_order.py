@@ -102,15 +102,15 @@ def test_per_box_orders_match_oracle(lfmm, orc, request, mixed):
 def test_learned_orders_gain_over_fixed(lfmm, orc, request):
     case_name(request)
     sets = []
-    for seed in (101, 102):
-        x, q = plummer(200000, seed)
-        feats, labels, info = lfmm.order.make_labels(lfmm, x, q, leaf_max=64, tol=1e-6, safety=0.5)
+    for seed, n_train in ((101, 200000), (102, 300000), (104, 400000)):
+        x, q = plummer(n_train, seed)
+        feats, labels, info = lfmm.order.make_labels(lfmm, x, q, leaf_max=64, tol=1e-6, safety=0.7)
         sets.append((feats, labels))
-        print("seed %d: %d leaves, label histogram %s" % (seed, len(labels), np.bincount(labels, minlength=13)),
-              flush=True)
+        print("seed %d: %d leaves, planned rel-L2 %.3g, label histogram %s" %
+              (seed, len(labels), info["planned_rel_l2"], np.bincount(labels, minlength=13)), flush=True)
     model, hist = lfmm.order.train(lfmm, sets, epochs=40)
     print("training loss %.4f -> %.4f" % (hist[0], hist[-1]), flush=True)
-    lfmm.set_order_model(model, confidence=0.9, check_tol=1e-6, check_samples=256)
+    lfmm.set_order_model(model, confidence=0.5, check_tol=1e-6, check_samples=256)
     n = 300000
     x, q = plummer(n, 103)
     idx = np.random.default_rng(5).choice(n, 2000, replace=False)
