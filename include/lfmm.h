@@ -44,7 +44,8 @@ typedef enum lfmm_status {
     LFMM_ERR_HIP = 5,          /* a HIP runtime call failed                */
     LFMM_ERR_NONFINITE = 6,    /* NaN or Inf in positions or charges       */
     LFMM_ERR_NODEVICE = 7,     /* no usable GPU                            */
-    LFMM_ERR_UNSUPPORTED = 8   /* option combination not implemented       */
+    LFMM_ERR_UNSUPPORTED = 8,  /* option combination not implemented       */
+    LFMM_ERR_TOO_LARGE = 9     /* input beyond a fixed scratch size         */
 } lfmm_status;
 
 typedef enum lfmm_memory {
@@ -312,6 +313,15 @@ int lfmm_tree2d(lfmm_ctx* ctx, const lfmm_points2d* particles, int tree_depth, i
  */
 int lfmm_direct3d(lfmm_ctx* ctx, const lfmm_points3d* sources, const lfmm_points3d* targets, int memory,
                   unsigned flags, double* potential, double* field);
+/* lfmm_direct3d_noalloc: lfmm_direct3d with device memory that allocates
+ * nothing (no reference counterpart): for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).  Its
+ * packed inputs go to a fixed 32 MiB array per GPU (1,048,576 points:
+ * sources plus targets; more returns LFMM_ERR_TOO_LARGE), held until the
+ * process ends.  All calls on one GPU share that array: calls that overlap
+ * in time on one GPU race.  Runs on ctx's stream and returns when done. */
+int lfmm_direct3d_noalloc(lfmm_ctx* ctx, const lfmm_points3d* sources, const lfmm_points3d* targets,
+                          unsigned flags, double* potential, double* field);
 int lfmm_direct2d(lfmm_ctx* ctx, const lfmm_points2d* sources, const lfmm_points2d* targets, int memory,
                   unsigned flags, double* potential, double* field);
 

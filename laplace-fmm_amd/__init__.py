@@ -28,7 +28,7 @@ STREAM_DEFAULT = 1  # LFMM_STREAM_DEFAULT: the null stream
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "expansion terms out of range", 3: "tree depth out of range",
           4: "out of device memory", 5: "HIP runtime error", 6: "non-finite input", 7: "no usable GPU",
-          8: "unsupported option"}
+          8: "unsupported option", 9: "input larger than a fixed scratch"}
 
 # Every extern "C" entry point declared in include/lfmm.h.
 EXPORTS = ("lfmm_create", "lfmm_destroy", "lfmm_strerror", "lfmm_set_stream", "lfmm_trim", "lfmm_fmm3d",
@@ -36,7 +36,8 @@ EXPORTS = ("lfmm_create", "lfmm_destroy", "lfmm_strerror", "lfmm_set_stream", "l
            "lfmm_host_m2l_dense", "lfmm_host_m2l16_pack", "lfmm_direct3d", "lfmm_direct2d", "lfmm_exact_filename",
            "lfmm_write_exact", "lfmm_read_exact", "lfmm_error_stats", "lfmm_fmm3d_vec", "lfmm_direct3d_vec",
            "lfmm_fmm3d_batch", "lfmm_tree3d_adaptive", "lfmm_fmm3d_orders", "lfmm_set_order_model",
-           "lfmm_order_features", "lfmm_order_predict", "lfmm_order_loss_grad", "lfmm_order_train")
+           "lfmm_order_features", "lfmm_order_predict", "lfmm_order_loss_grad", "lfmm_order_train",
+           "lfmm_direct3d_noalloc")
 
 
 class LfmmError(RuntimeError):
@@ -152,6 +153,9 @@ def lib():
         L.lfmm_direct3d_vec.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points3d), ctypes.POINTER(Points3d),
                                         ctypes.c_int, ctypes.c_void_p]
         L.lfmm_direct3d_vec.restype = ctypes.c_int
+        L.lfmm_direct3d_noalloc.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points3d), ctypes.POINTER(Points3d),
+                                            ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p]
+        L.lfmm_direct3d_noalloc.restype = ctypes.c_int
         L.lfmm_tree3d_adaptive.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points3d), ctypes.c_int, ctypes.c_int] + \
             [ctypes.c_void_p] * 13
         L.lfmm_tree3d_adaptive.restype = ctypes.c_int

@@ -8,6 +8,7 @@
 // Everything runs on one HIP stream; the only host synchronisations are the
 // box counts of each level (needed to size the next launch).
 #include "api_internal.h"
+#include "direct.h"
 
 #include <hip/hip_runtime.h>
 
@@ -996,6 +997,11 @@ int lfmm_create(lfmm_ctx** out, int device)
     lfmm_ctx* c = new (std::nothrow) lfmm_ctx();
     if (!c) return LFMM_ERR_NOMEM;
     c->device = device;
+    // this device's address of the allocation-free entry point's scratch
+    if (noalloc_scratch_init(device) != hipSuccess) {
+        delete c;
+        return LFMM_ERR_HIP;
+    }
     if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return LFMM_ERR_HIP;
@@ -1036,6 +1042,7 @@ const char* lfmm_strerror(int s)
     case LFMM_ERR_NONFINITE: return "non-finite input";
     case LFMM_ERR_NODEVICE: return "no usable GPU";
     case LFMM_ERR_UNSUPPORTED: return "unsupported option";
+    case LFMM_ERR_TOO_LARGE: return "input larger than the fixed scratch of an allocation-free entry point";
     default: return "unknown status";
     }
 }

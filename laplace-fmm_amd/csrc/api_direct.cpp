@@ -230,4 +230,39 @@ int lfmm_error_stats(int kind, int64_t n, const double* exact, const double* res
     return LFMM_OK;
 }
 
+// The allocation-free variant of lfmm_direct3d (device memory): the packed
+// records go to the fixed per-GPU scratch array of direct.hip instead of the
+// context's workspace.  Calls overlapping on one GPU share that array and race.
+int lfmm_direct3d_noalloc(lfmm_ctx* c, const lfmm_points3d* sources, const lfmm_points3d* targets, unsigned flags,
+                          double* potential, double* field)
+{
+    if (!c || !valid_points(sources, true, 3)) return LFMM_ERR_INVALID;
+    if (targets && !valid_points(targets, false, 3)) return LFMM_ERR_INVALID;
+    const lfmm_points3d* tg = (targets && targets->n > 0) ? targets : nullptr;
+    const int64_t ns = sources->n, nt = tg ? tg->n : ns;
+    if (nt == 0) return LFMM_OK;
+    if (ns + (tg ? nt : 0) > (int64_t)(kNoAllocScratchBytes / sizeof(double4))) return LFMM_ERR_TOO_LARGE;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return LFMM_ERR_HIP;
+    double4* scr = noalloc_scratch(c->device);
+    if (!scr) return LFMM_ERR_HIP;
+    hipStream_t st = c->stream;
+    if (ns == 0) {  // no sources: every output is zero
+        if (potential) LFMM_HIP(fill_u32((uint32_t*)potential, 2 * nt, 0u, st));
+        if (field) LFMM_HIP(fill_u32((uint32_t*)field, 6 * nt, 0u, st));
+        LFMM_HIP(hipStreamSynchronize(st));
+        return LFMM_OK;
+    }
+    LFMM_HIP(pack_points(3, sources->pos, sources->pos_stride, sources->q, sources->q_stride, ns, scr, st));
+    const double4* tgt = scr;
+    if (tg) {
+        LFMM_HIP(pack_points(3, tg->pos, tg->pos_stride, nullptr, 1, nt, scr + ns, st));
+        tgt = scr + ns;
+    }
+    const bool exclude = !tg && (flags & LFMM_FLAG_EXCLUDE_SELF);
+    LFMM_HIP(launch_direct(3, tgt, nt, scr, ns, exclude, potential, field, st));
+    LFMM_HIP(hipStreamSynchronize(st));
+    return LFMM_OK;
+}
+
 }  // extern "C"

@@ -14,4 +14,13 @@ hipError_t launch_direct(int dim, const double4* tgt, int64_t nt, const double4*
 // x, y, z components (n x 3 doubles each).
 hipError_t launch_cross_fields(const double* e1, const double* e2, double* e3, int64_t n, hipStream_t st);
 
+// Fixed scratch of lfmm_direct3d_noalloc: 32 MiB per GPU, 1,048,576 packed
+// records (sources plus targets of one call).
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+// Looks up this device's copy of the scratch array (with `device` current);
+// once per device, at context creation.  noalloc_scratch returns it (null
+// before the lookup).
+hipError_t noalloc_scratch_init(int device);
+double4* noalloc_scratch(int device);
+
 }  // namespace lfmm

@@ -12,6 +12,8 @@
 #include "device_common.h"
 #include "direct.h"
 
+#include <mutex>
+
 namespace lfmm {
 
 namespace {
@@ -140,6 +142,41 @@ hipError_t launch_direct(int dim, const double4* tgt, int64_t nt, const double4*
     else
         hipLaunchKernelGGL(k_direct2d, grid, dim3(kTile), 0, st, tgt, nt, src, ns, exclude_self ? 1 : 0, pot, field);
     return hipGetLastError();
+}
+
+// ---- the allocation-free direct sum (lfmm_direct3d_noalloc)
+// The packed (x, y, z, q) records of one call's sources and targets live in
+// this fixed array, one copy per GPU, held until the process ends.  Host code
+// never passes the array's name to a kernel (in host code it is a stand-in,
+// not a device address): each device's address comes from
+// hipGetSymbolAddress once, with that device current (noalloc_scratch_init,
+// from lfmm_create), and is used from then on.
+constexpr int64_t kNoAllocRecords = (int64_t)(kNoAllocScratchBytes / sizeof(double4));
+static __device__ __align__(256) double4 scratch[kNoAllocRecords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double4* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = (double4*)p;
+    return hipSuccess;
+}
+
+double4* noalloc_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    return g_scratch_ptr[device];
 }
 
 }  // namespace lfmm
