@@ -17,10 +17,17 @@ p = 10 configuration (11 terms) and the all-FP64 path at both orders, each
 with its own measured error, all against one direct-sum reference on the same
 sampled targets.
 
+Also reported: the Plummer sphere (4M particles, scale radius 0.05) on the
+adaptive octree -- fixed-order FP64, fixed-order mixed, and per-box orders
+from the order network trained in this run on the GPU -- the 2D 1M
+configuration, 512 independent systems, and CPU baselines (the CPU oracle,
+median of 5 solves after a warm-up).
+
 Printed: one JSON line with particles/s, the accuracy, the M2L roofline of
-the headline (kernel time from HIP events recorded by the library on its
-stream during the timed steps) and a CPU baseline (the CPU oracle on a
-bounded sample).
+the headline on its algorithmic flop count (kernel time from HIP events the
+library records on its stream during the timed steps), the issued matrix-core
+rate beside it, HBM rates of the P2P, M2M and L2L kernels, and the CPU
+baseline.
 """
 import argparse
 import json
@@ -36,12 +43,13 @@ sys.path.insert(0, ROOT)
 METRIC = "particles/sec (3D, 4M) at rel-L2 err ≤1e-6 vs FP64 direct; M2L MFMA util"
 FP64_PEAK_TFLOPS = 78.6
 FP16_PEAK_TFLOPS = 2500.0  # dense MFMA
+HBM_PEAK_GBS = 8000.0
 
 
 def m2l_flops_per_op(N):
     """Algorithmic FP64 flops of one rotation-based M2L with N terms:
     Rz and Ry forward and back (2 (6 Nx2 + sum_{n<N} (n+1)(4n+2))) plus the
-    axial translation (4 sum_{t=1..N} t^2)."""
+    axial translation (4 sum_{t=1..N} t^2).  10,556 at N = 13."""
     nx2 = N * (N + 1) // 2
     rot = sum((n + 1) * (4 * n + 2) for n in range(N))
     return 2 * (6 * nx2 + rot) + 4 * sum(t * t for t in range(1, N + 1))
@@ -96,6 +104,21 @@ def batch_systems(n_sys, seed):
     return np.concatenate(xs), np.concatenate(qs), off
 
 
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+def median_time(fn, reps=5):
+    """Median wall time of reps calls after one warm-up call."""
+    fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,9 +132,18 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--sample", type=int, default=4096, help="targets checked against the direct sum")
     ap.add_argument("--cpu-n", type=int, default=0, help="particles in the CPU-baseline run (0: the full workload)")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="timed CPU solves per configuration (median)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary configurations")
+    ap.add_argument("--no-plummer", action="store_true")
+    ap.add_argument("--no-uniform-plummer", action="store_true",
+                    help="skip the fixed-depth Plummer solve (2.7 s per solve)")
+    ap.add_argument("--workload", choices=("uniform", "plummer"), default="uniform",
+                    help="particles of the timed steps (profiling runs): the uniform cube of the metric, or the "
+                         "Plummer sphere on the adaptive octree")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="one device copy of this many MiB before the steps (calibrates HBM counters)")
     args = ap.parse_args()
     mixed = not args.fp64
 
@@ -129,6 +161,9 @@ def main():
     rng = np.random.default_rng(args.seed)
     x = rng.random((args.n, 3))
     q = rng.uniform(-1.0, 1.0, args.n)
+    adaptive = args.workload == "plummer"
+    if adaptive:  # self term excluded on this workload (sampled errors need no correction)
+        x, q = plummer(args.n, args.seed + 11)
     xd = torch.from_numpy(x).to(dev)
     qd = torch.from_numpy(q).to(dev)
     pot = torch.empty(args.n, dtype=torch.float64, device=dev)
@@ -137,10 +172,20 @@ def main():
     idx = np.random.default_rng(args.seed + 1).choice(args.n, min(args.sample, args.n), replace=False)
     idx_d = torch.from_numpy(idx).to(dev)
 
+    if args.copy_mib > 0:
+        a_ = torch.empty(args.copy_mib << 17, dtype=torch.float64, device=dev)
+        b_ = torch.empty_like(a_)
+        b_.copy_(a_)
+        torch.cuda.synchronize()
+        del a_, b_
+
     def run(terms, mix, steps, warmup):
-        def step(timings=False):
+        def step(timings=False, count=False):
+            if adaptive:
+                return lfmm.fmm3d(xd, qd, terms=terms, leaf_max=64, adaptive=True, exclude_self=True, ctx=ctx,
+                                  out=(pot, field), timings=timings, count_ops=count, mixed=mix)[2]
             return lfmm.fmm3d(xd, qd, terms=terms, tree_depth=args.depth, ctx=ctx, out=(pot, field),
-                              timings=timings, mixed=mix)[2]
+                              timings=timings, count_ops=count, mixed=mix)[2]
 
         for _ in range(warmup):
             step()
@@ -152,55 +197,83 @@ def main():
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
         sample = (pot[idx_d].cpu().numpy(), field[idx_d].cpu().numpy())
-        print("bench: terms=%d %s: %d steps, %.3f ms/step" % (terms, "mixed" if mix else "fp64", steps,
-                                                              1e3 * elapsed / steps), file=sys.stderr, flush=True)
+        log("terms=%d %s: %d steps, %.3f ms/step" % (terms, "mixed" if mix else "fp64", steps, 1e3 * elapsed / steps))
         return {"ms": 1e3 * elapsed / steps, "value": args.n / (elapsed / steps), "reports": reports,
-                "sample": sample}
+                "sample": sample, "step": step}
+
+    def mean_of(reports, key):
+        return float(np.mean([r[key] for r in reports]))
 
     head = run(args.terms, mixed, args.steps, args.warmup)
     # exact op counts (after the timed region)
-    counts = lfmm.fmm3d(xd, qd, terms=args.terms, tree_depth=args.depth, ctx=ctx, out=(pot, field),
-                        timings=True, count_ops=True, mixed=mixed)[2]
+    counts = head["step"](timings=True, count=True)
     reports = head["reports"]
-    m2l_ms = float(np.mean([r["t_m2l_kernels_ms"] for r in reports]))
-    leaf_ms = float(np.mean([r["t_leaf_kernel_ms"] for r in reports]))
+    m2l_ms = mean_of(reports, "t_m2l_kernels_ms")
+    leaf_ms = mean_of(reports, "t_leaf_kernel_ms")
+    alg = m2l_flops_per_op(args.terms)
     if mixed:
-        # dominant kernel: the matrix-core M2L of levels >= 4
-        mx_ms = float(np.mean([r["t_m2l_mixed_ms"] for r in reports]))
+        # dominant kernel: the matrix-core M2L of levels >= 4, on its
+        # algorithmic (rotation-based) flop count; the MFMA work it issues
+        # (dense operators, three split products) beside it
+        mx_ms = mean_of(reports, "t_m2l_mixed_ms")
         mx_launches = reports[0]["m2l_mixed_launches"]
         mx_ops = counts["m2l_ops_mixed"]
-        fpo = m2l16_mfma_flops_per_op(lfmm, args.terms)
-        ach = mx_ops * fpo / (mx_ms * 1e-3) / 1e12
+        ach = mx_ops * alg / (mx_ms * 1e-3) / 1e12
+        issued = mx_ops * m2l16_mfma_flops_per_op(lfmm, args.terms) / (mx_ms * 1e-3) / 1e12
         roofline = {"kernel": "k_m2l16r<N> (split-FP16 MFMA M2L, levels >= 4, one launch per level)", "bound": "mfma",
                     "achieved": round(ach, 3), "peak": FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(ach / FP16_PEAK_TFLOPS, 4), "traffic": None,
-                    "m2l_ops": mx_ops, "flops_per_op": round(fpo, 1), "flops_counted": "MFMA tile flops issued",
-                    "launches": mx_launches, "avg_launch_ms": round(mx_ms / max(mx_launches, 1), 4),
-                    "m2l_kernels_ms_per_step": round(mx_ms, 4),
+                    "flops_counted": "algorithmic rotation-based M2L flops (m2l_ops x %d)" % alg,
+                    "m2l_ops": mx_ops, "flops_per_op": alg, "launches": mx_launches,
+                    "avg_launch_ms": round(mx_ms / max(mx_launches, 1), 4), "m2l_kernels_ms_per_step": round(mx_ms, 4),
+                    "mfma_issued": {"tflops": round(issued, 1), "frac_of_peak": round(issued / FP16_PEAK_TFLOPS, 4),
+                                    "flops_per_op": round(m2l16_mfma_flops_per_op(lfmm, args.terms), 1)},
                     "precision": "fp16 hi/lo operands (22 bits), fp32 accumulation",
-                    "fp64_levels": {"m2l_ops": counts["m2l_ops"] - mx_ops, "ms_per_step": round(m2l_ms - mx_ms, 4)},
-                    "fp64_equivalent_tflops": round(mx_ops * m2l_flops_per_op(args.terms) / (mx_ms * 1e-3) / 1e12, 3)}
+                    "fp64_levels": {"m2l_ops": counts["m2l_ops"] - mx_ops, "ms_per_step": round(m2l_ms - mx_ms, 4)}}
     else:
         launches = reports[0]["m2l_launches"]
-        ach = counts["m2l_ops"] * m2l_flops_per_op(args.terms) / (m2l_ms * 1e-3) / 1e12
+        ach = counts["m2l_ops"] * alg / (m2l_ms * 1e-3) / 1e12
         roofline = {"kernel": "k_rtr<N,M2L> (M2L+L2L, one launch per level)", "bound": "mfma",
                     "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                    "m2l_ops": counts["m2l_ops"], "flops_per_op": m2l_flops_per_op(args.terms),
-                    "launches": launches, "avg_launch_ms": round(m2l_ms / max(launches, 1), 4),
+                    "flops_counted": "algorithmic rotation-based M2L flops", "m2l_ops": counts["m2l_ops"],
+                    "flops_per_op": alg, "launches": launches, "avg_launch_ms": round(m2l_ms / max(launches, 1), 4),
                     "m2l_kernels_ms_per_step": round(m2l_ms, 4), "precision": "fp64 VALU"}
-    tf = os.path.join(ROOT, "profiles", "m2l_traffic.json")
-    if os.path.exists(tf):
+    prof = os.path.join(ROOT, "profiles", "m2l_traffic.json")
+    if os.path.exists(prof):
         try:
-            t = json.load(open(tf))
+            t = json.load(open(prof))
             if (t.get("n") == args.n and t.get("terms") == args.terms and t.get("tree_depth") == args.depth
                     and bool(t.get("mixed", False)) == mixed):
                 roofline["traffic"] = t.get("hbm_bytes_per_launch")
+                roofline["traffic_source"] = "profiles/m2l_traffic.json (rocprofv3 FETCH_SIZE, WRITE_SIZE)"
+                if t.get("mfma_busy") is not None:
+                    roofline["mfma_busy_level6"] = t.get("mfma_busy")
         except (OSError, ValueError):
             pass
+    # HBM rates of the P2P, M2M and L2L kernels on their algorithmic bytes
+    n_out = args.n
+    p2p_bytes = 32.0 * counts["p2p_source_reads"] + args.n * (32.0 + 32.0) + n_out * 32.0
+
+    def rate(nbytes, ms):
+        gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else None
+        return {"bytes": nbytes, "ms_per_step": round(ms, 4), "gbs": round(gbs, 1) if gbs else None,
+                "frac_of_hbm": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}
+    hbm = {"peak_gbs": HBM_PEAK_GBS,
+           "p2p": dict(rate(p2p_bytes, mean_of(reports, "t_p2p_ms")),
+                       kernel="k_p2p_multi<2> with its lane setup",
+                       bytes_counted="32 B per near-list source per leaf + 32 B target + 32 B far field + 32 B out"),
+           "m2m": dict(rate(counts["m2m_bytes"], mean_of(reports, "t_m2m_ms")), kernel="k_rtr<N,M2M>",
+                       bytes_counted="every multipole of levels >= 3 read once, levels >= 2 written once"),
+           "l2l": dict(rate(counts["l2l_bytes"], mean_of(reports, "t_l2l_ms")), kernel="k_rtr<N,L2L>",
+                       bytes_counted="parent locals read once, child locals written once (levels >= 4)")}
     p2p_pairs = counts["p2p_pairs"]
     leaf = {"kernel": "k_l2p<N> + k_p2p (L2P, then P2P)", "ms_per_step": round(leaf_ms, 4), "p2p_pairs": p2p_pairs,
-            "gflops_p2p": round(20.0 * p2p_pairs / (leaf_ms * 1e-3) / 1e9, 1)}
+            "p2p_gflops": round(20.0 * p2p_pairs / (mean_of(reports, "t_p2p_ms") * 1e-3) / 1e9, 1),
+            "p2p_frac_of_fp64_peak": round(20.0 * p2p_pairs / (mean_of(reports, "t_p2p_ms") * 1e-3) / 1e12 /
+                                           FP64_PEAK_TFLOPS, 4)}
+    stages = {k: round(mean_of(reports, k), 4) for k in ("t_p2m_ms", "t_m2m_ms", "t_l2l_ms", "t_m2l_kernels_ms",
+                                                         "t_l2p_ms", "t_p2p_ms")}
 
     # secondary configurations: p = 10 (11 terms) and the all-FP64 path
     extras = []
@@ -209,31 +282,22 @@ def main():
             if (terms, mix) == (args.terms, mixed):
                 continue
             r = run(terms, mix, 3, 1)
-            extras.append({"terms": terms, "m2l": "split-FP16 MFMA (levels >= 4)" if mix else "FP64", "r": r})
+            c = r["step"](count=True)
+            e = {"terms": terms, "m2l": "split-FP16 MFMA (levels >= 4)" if mix else "FP64", "r": r}
+            if not mix:  # the FP64 path's own fraction of its peak (FP64 VALU = FP64 MFMA = 78.6 TF)
+                tm = mean_of(r["reports"], "t_m2l_kernels_ms")
+                tf = c["m2l_ops"] * m2l_flops_per_op(terms) / (tm * 1e-3) / 1e12
+                e["m2l_fp64"] = {"tflops": round(tf, 2), "frac_of_fp64_peak": round(tf / FP64_PEAK_TFLOPS, 4),
+                                 "m2l_kernels_ms": round(tm, 3)}
+            extras.append(e)
+        roofline["fp64_path"] = next((e["m2l_fp64"] for e in extras if e["terms"] == args.terms and "m2l_fp64" in e),
+                                     None)
 
-    # Plummer sphere (the north star's distribution): fixed order, uniform
-    # depth from leaf_max 64, FP64 against the matrix-core path
-    plum = []
-    if not args.no_extra:
-        xp_, qp_ = plummer(args.n, args.seed + 11)
-        xpd, qpd = torch.from_numpy(xp_).to(dev), torch.from_numpy(qp_).to(dev)
-        for pterms, mix in ((args.terms, False), (args.terms, True), (args.terms + 1, True)):
-            def pstep():
-                return lfmm.fmm3d(xpd, qpd, terms=pterms, leaf_max=64, ctx=ctx, out=(pot, field), mixed=mix)[2]
-            rep = pstep()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(3):
-                pstep()
-            torch.cuda.synchronize()
-            ms = 1e3 * (time.perf_counter() - t0) / 3
-            plum.append({"terms": pterms, "m2l": "split-FP16 MFMA (levels >= 4)" if mix else "FP64",
-                         "tree_depth": rep["tree_depth"], "max_leaf": rep["max_leaf"], "ms_per_step": round(ms, 3),
-                         "value": args.n / (ms * 1e-3),
-                         "sample": (pot[idx_d].cpu().numpy(), field[idx_d].cpu().numpy())})
-            print("bench: plummer terms=%d %s depth %d: %.3f ms/step" % (pterms, "mixed" if mix else "fp64",
-                                                                       rep["tree_depth"], ms), file=sys.stderr,
-                  flush=True)
+    # Plummer sphere on the adaptive octree: fixed-order FP64 (the plain
+    # method), fixed-order mixed, and learned per-box orders (mixed)
+    plum = None
+    if not args.no_extra and not args.no_plummer:
+        plum = plummer_leg(args, lfmm, torch, ctx, dev, idx, idx_d, pot, field)
 
     # 2D, 1,048,576 uniform points, p = 20 (21 terms), leaf_max 64: FP64
     # against FP32 per-direction M2L sums at levels >= 4
@@ -255,8 +319,7 @@ def main():
             runs2d.append({"m2l": "FP32 sums at levels >= 4, FP64 accumulation" if mix else "FP64",
                            "tree_depth": rep2["tree_depth"], "ms_per_step": round(ms2, 3),
                            "value": 1048576 / (ms2 * 1e-3), "sample": (p2_.cpu().numpy(), f2_.cpu().numpy())})
-            print("bench: 2d 1M p=20 %s %.3f ms/step" % ("fp32-m2l" if mix else "fp64", ms2), file=sys.stderr,
-                  flush=True)
+            log("2d 1M p=20 %s %.3f ms/step" % ("fp32-m2l" if mix else "fp64", ms2))
 
     # many independent systems in one call (p = 8, leaf_max 32), against a
     # loop of single solves over the same device-resident systems
@@ -290,8 +353,7 @@ def main():
                  "particles": int(len(qb)), "ms_per_batch": round(bms, 3), "value": len(qb) / (bms * 1e-3),
                  "unit": "particles/s", "loop_of_single_solves_ms": round(sms, 3),
                  "speedup_vs_loop": round(sms / bms, 2), "workers": 8}
-        print("bench: batch of 512 systems %.3f ms (loop of single solves %.3f ms)" % (bms, sms), file=sys.stderr,
-              flush=True)
+        log("batch of 512 systems %.3f ms (loop of single solves %.3f ms)" % (bms, sms))
 
     accuracy = None
     other = []
@@ -300,12 +362,12 @@ def main():
         import oracle
         t1 = time.perf_counter()
         dp, df = oracle.direct3d(x, q, tgt_index=idx, include_self=False)
-        print("bench: direct sum on %d targets took %.1f s" % (len(idx), time.perf_counter() - t1), file=sys.stderr,
-              flush=True)
+        log("direct sum on %d targets took %.1f s" % (len(idx), time.perf_counter() - t1))
 
         def errors(sample):
             p, f = sample
-            phys = p - q[idx] * 1e4  # remove the reference's clamped self term q_i / 1e-4
+            # remove the reference's clamped self term q_i / 1e-4 (the Plummer workload excludes it)
+            phys = p if adaptive else p - q[idx] * 1e4
             return (float(np.linalg.norm(phys - dp) / np.linalg.norm(dp)),
                     float(np.linalg.norm(f - df) / np.linalg.norm(df)))
 
@@ -314,17 +376,40 @@ def main():
                     "meets_1e-6": bool(ep <= 1e-6),
                     "reference_convention": dict(zip(("max_err", "l2_err", "rel_err"),
                                                      map(float, oracle.read_error(dp + q[idx] * 1e4,
-                                                                                  head["sample"][0]))))}
+                                                                                  head["sample"][0] +
+                                                                                  (q[idx] * 1e4 if adaptive else 0)))))}
         for e in extras:
             e["rel_l2_potential"], e["rel_l2_field"] = errors(e["r"]["sample"])
-        if plum:
+        if plum is not None:
             t1 = time.perf_counter()
-            pdp, pdf = oracle.direct3d(xp_, qp_, tgt_index=idx, include_self=False)
-            print("bench: plummer direct sum took %.1f s" % (time.perf_counter() - t1), file=sys.stderr, flush=True)
-            for e in plum:
-                p_, f_ = e["sample"]
-                e["rel_l2_potential"] = float(np.linalg.norm(p_ - qp_[idx] * 1e4 - pdp) / np.linalg.norm(pdp))
+            pdp, pdf = oracle.direct3d(plum["x"], plum["q"], tgt_index=idx, include_self=False)
+            log("plummer direct sum took %.1f s" % (time.perf_counter() - t1))
+            for e in plum["runs"]:
+                p_, f_ = e.pop("sample")
+                e["rel_l2_potential"] = float(np.linalg.norm(p_ - pdp) / np.linalg.norm(pdp))
                 e["rel_l2_field"] = float(np.linalg.norm(f_ - pdf) / np.linalg.norm(pdf))
+                e["meets_1e-6"] = bool(e["rel_l2_potential"] <= 1e-6)
+    if plum is not None:
+        for e in plum["runs"]:
+            e.pop("sample", None)
+        plum.pop("x")
+        plum.pop("q")
+        runs = plum["runs"]
+
+        def fastest(pred):
+            ok = [e for e in runs if pred(e) and e.get("meets_1e-6")]
+            return min(ok, key=lambda e: e["ms_per_step"]) if ok else None
+        base = fastest(lambda e: e["m2l"] == "FP64" and e["tree"] == "adaptive" and e["orders"] == "fixed")
+        base_u = fastest(lambda e: e["m2l"] == "FP64" and e["tree"] == "uniform depth")
+        learned = fastest(lambda e: e["orders"] == "learned")
+        fixed_mx = fastest(lambda e: e["m2l"] != "FP64" and e["orders"] == "fixed" and e["tree"] == "adaptive")
+        plum["speedup_at_1e-6"] = {
+            "learned_mixed_vs_fixed_fp64_adaptive": round(base["ms_per_step"] / learned["ms_per_step"], 2)
+            if base and learned else None,
+            "learned_mixed_vs_fixed_fp64_uniform_depth": round(base_u["ms_per_step"] / learned["ms_per_step"], 2)
+            if base_u and learned else None,
+            "fixed_mixed_vs_fixed_fp64_adaptive": round(base["ms_per_step"] / fixed_mx["ms_per_step"], 2)
+            if base and fixed_mx else None}
     if runs2d and not args.no_error:
         i2 = np.random.default_rng(args.seed + 32).choice(1048576, 1024, replace=False)
         dp2, df2 = oracle.direct2d(x2, q2, tgt_index=i2, include_self=False)
@@ -347,57 +432,147 @@ def main():
             efs_.append(float(np.linalg.norm(fbh[a_:b_][ix] - df_) / np.linalg.norm(df_)))
         batch["rel_l2_potential_max_over_8_systems"] = max(eps_)
         batch["rel_l2_field_max_over_8_systems"] = max(efs_)
-    for e in plum:
-        e.pop("sample")
-        e["meets_1e-6"] = bool(e.get("rel_l2_potential", 1.0) <= 1e-6)
-    # fastest matrix-core run against the fastest FP64 run, both at rel-L2 <= 1e-6
-    ok64 = [e["ms_per_step"] for e in plum if e["m2l"] == "FP64" and e["meets_1e-6"]]
-    okmx = [e["ms_per_step"] for e in plum if e["m2l"] != "FP64" and e["meets_1e-6"]]
-    plum_speedup = round(min(ok64) / min(okmx), 2) if ok64 and okmx else None
     for e in extras:
         r = e.pop("r")
         e.update({"ms_per_step": round(r["ms"], 3), "value": r["value"],
                   "meets_1e-6": bool(e.get("rel_l2_potential", 1.0) <= 1e-6)})
         other.append(e)
 
-    cpu = None
+    cpu, cpu_all = None, None
     if not args.no_cpu_baseline:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle
-        cn = args.cpu_n if args.cpu_n > 0 else args.n
-        if cn == args.n:  # the same particles
-            xs, qs = x, q
-        else:
-            rs = np.random.default_rng(args.seed + 7)
-            xs = rs.random((cn, 3))
-            qs = rs.uniform(-1.0, 1.0, cn)
-        cdepth = max(2, args.depth - int(round(np.log(args.n / cn) / np.log(8.0))))
-        t1 = time.perf_counter()
-        oracle.fmm3d(xs, qs, args.terms, cdepth)
-        tc = time.perf_counter() - t1
-        cpu = {"value": cn / tc, "unit": "particles/s", "cores": oracle.num_threads(), "kind": "port",
-               "sample": "CPU oracle (C/OpenMP restatement of the reference CPU FMM, FP64) on %d uniform particles, "
-                         "%d terms, depth %d, one solve of %.1f s" % (cn, args.terms, cdepth, tc)}
-        print("bench: cpu baseline %.1f s" % tc, file=sys.stderr, flush=True)
+        cpu, cpu_all = cpu_baselines(args, x, q)
 
     out = {"metric": METRIC, "value": head["value"], "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": head["ms"], "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64+f16x2" if mixed else "f64", "data": "synthetic",
-           "config": {"workload": "3D Laplace FMM, %d particles uniform in [0,1)^3, q~U[-1,1], p=%d (%d terms), "
-                                  "uniform octree depth %d (leaves at level %d)" % (args.n, args.terms - 1, args.terms,
-                                                                                     args.depth, args.depth - 1),
+           "config": {"workload": ("3D Laplace FMM, %d particles uniform in [0,1)^3, q~U[-1,1], p=%d (%d terms), "
+                                   "uniform octree depth %d (leaves at level %d)" % (args.n, args.terms - 1, args.terms,
+                                                                                      args.depth, args.depth - 1))
+                      if not adaptive else
+                      ("3D Laplace FMM, %d particles in a Plummer sphere (a = 0.05, r <= 1), q~U[-1,1], p=%d "
+                       "(%d terms), adaptive octree leaf_max 64" % (args.n, args.terms - 1, args.terms)),
                       "n": args.n, "terms": args.terms, "tree_depth": args.depth, "max_leaf": counts["max_leaf"],
                       "m2l": ("split-FP16 MFMA at levels >= 4, FP64 at levels 2-3" if mixed else "FP64"),
                       "parallelism": "single GPU"},
-           "accuracy": accuracy, "roofline": roofline, "leaf_kernel": leaf, "other_configs": other,
-           "plummer": {"workload": "%d particles, Plummer sphere a = 0.05 truncated at r = 1, centred in the unit "
-                                   "cube (bounding box grows), q~U[-1,1], uniform depth from leaf_max 64" % args.n,
-                       "runs": plum, "mixed_speedup_at_1e-6": plum_speedup} if plum else None,
+           "accuracy": accuracy, "roofline": roofline, "hbm": hbm, "stage_ms": stages, "leaf_kernel": leaf,
+           "other_configs": other, "plummer": plum,
            "config2d": {"workload": "2D Laplace FMM, 1048576 points uniform in [0,1)^2, q~U[-1,1], p=20 (21 terms), "
                                     "leaf_max 64, self term excluded", "runs": runs2d} if runs2d else None,
-           "batch": batch,
-           "cpu_baseline": cpu}
+           "batch": batch, "cpu_baseline": cpu, "cpu_baselines": cpu_all}
     print(json.dumps(out), flush=True)
+
+
+def plummer_leg(args, lfmm, torch, ctx, dev, idx, idx_d, pot, field):
+    """4M-particle Plummer sphere (a = 0.05, truncated at r = 1, centred in
+    the unit cube; the points outside it are kept, the bounding box grows) on
+    the adaptive octree (leaf_max 64): fixed orders in FP64 and mixed, and
+    per-box orders from the order network trained here on the GPU on other
+    Plummer spheres (labels from this FMM at every order).  Samples are
+    taken without the self term (exclude_self)."""
+    xp, qp = plummer(args.n, args.seed + 11)
+    xpd, qpd = torch.from_numpy(xp).to(dev), torch.from_numpy(qp).to(dev)
+    runs = []
+
+    def timed(label, fn, steps=3):
+        rep = fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            rep = fn()
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        log("plummer %s: %.3f ms/step" % (label, ms))
+        return ms, rep
+
+    for terms, mix in ((args.terms, False), (args.terms, True), (args.terms + 1, True)):
+        ms, rep = timed("adaptive terms=%d %s" % (terms, "mixed" if mix else "fp64"),
+                        lambda: lfmm.fmm3d(xpd, qpd, terms=terms, leaf_max=64, adaptive=True, exclude_self=True,
+                                           ctx=ctx, out=(pot, field), mixed=mix)[2])
+        runs.append({"tree": "adaptive", "orders": "fixed", "terms": terms,
+                     "m2l": "split-FP16 MFMA (levels >= 4)" if mix else "FP64",
+                     "tree_depth": rep["tree_depth"], "leaves": rep["num_leaves"], "max_leaf": rep["max_leaf"],
+                     "ms_per_step": round(ms, 3), "value": args.n / (ms * 1e-3),
+                     "sample": (pot[idx_d].cpu().numpy(), field[idx_d].cpu().numpy())})
+    # learned per-box orders: training data from three other spheres
+    t0 = time.perf_counter()
+    sets, label_info = [], []
+    for k, n_train in enumerate((1048576, 2097152, args.n)):
+        xt, qt = plummer(n_train, args.seed + 101 + k)
+        f_, y_, info = lfmm.order.make_labels(lfmm, xt, qt, leaf_max=64, tol=1e-6, safety=0.7, mixed=True, ctx=ctx)
+        sets.append((f_, y_))
+        label_info.append({"n": n_train, "leaves": info["leaves"], "planned_rel_l2": info["planned_rel_l2"],
+                           "label_histogram": np.bincount(y_, minlength=13).tolist()})
+    t_labels = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    model, hist = lfmm.order.train(lfmm, sets, epochs=40, ctx=ctx)
+    t_train = time.perf_counter() - t0
+    log("order network: labels %.1f s, training %.1f s, loss %.3f -> %.3f" % (t_labels, t_train, hist[0], hist[-1]))
+    lfmm.set_order_model(model, confidence=0.5, check_tol=1e-6, check_samples=256, ctx=ctx)
+    ms, rep = timed("adaptive learned orders mixed",
+                    lambda: lfmm.fmm3d(xpd, qpd, leaf_max=64, learned_order=True, exclude_self=True, ctx=ctx,
+                                       out=(pot, field), mixed=True)[2])
+    runs.append({"tree": "adaptive", "orders": "learned", "terms": [rep["terms_min"], rep["terms"]],
+                 "m2l": "split-FP16 MFMA (levels >= 4)", "tree_depth": rep["tree_depth"], "leaves": rep["num_leaves"],
+                 "max_leaf": rep["max_leaf"], "ms_per_step": round(ms, 3), "value": args.n / (ms * 1e-3),
+                 "check_rel_l2": rep["check_rel_l2"], "fallback": rep["fallback"],
+                 "sample": (pot[idx_d].cpu().numpy(), field[idx_d].cpu().numpy())})
+    fl = lfmm.fmm3d(xpd, qpd, leaf_max=64, learned_order=True, exclude_self=True, ctx=ctx, out=(pot, field),
+                    mixed=True, count_ops=True)[2]
+    fx = lfmm.fmm3d(xpd, qpd, terms=args.terms, leaf_max=64, adaptive=True, exclude_self=True, ctx=ctx,
+                    out=(pot, field), mixed=True, count_ops=True)[2]
+    runs[-1]["m2l_flops"] = fl["m2l_flops"]
+    runs[-1]["m2l_flops_vs_fixed_%d_terms" % args.terms] = round(fl["m2l_flops"] / fx["m2l_flops"], 3)
+    lfmm.set_order_model(None, ctx=ctx)
+    if not args.no_uniform_plummer:  # the reference's kind of tree: one depth for all leaves
+        ms, rep = timed("uniform depth terms=%d fp64" % args.terms,
+                        lambda: lfmm.fmm3d(xpd, qpd, terms=args.terms, leaf_max=64, exclude_self=True, ctx=ctx,
+                                           out=(pot, field))[2], steps=1)
+        runs.append({"tree": "uniform depth", "orders": "fixed", "terms": args.terms, "m2l": "FP64",
+                     "tree_depth": rep["tree_depth"], "leaves": rep["num_leaves"], "max_leaf": rep["max_leaf"],
+                     "ms_per_step": round(ms, 3), "value": args.n / (ms * 1e-3),
+                     "sample": (pot[idx_d].cpu().numpy(), field[idx_d].cpu().numpy())})
+    return {"workload": "%d particles, Plummer sphere a = 0.05 truncated at r = 1, centred in the unit cube "
+                        "(bounding box grows), q~U[-1,1], adaptive octree leaf_max 64, self term excluded" % args.n,
+            "runs": runs, "x": xp, "q": qp,
+            "order_network": {"features": 8, "hidden": [64, 64], "classes": 13, "training_sets": label_info,
+                              "epochs": len(hist), "loss_first_last": [float(hist[0]), float(hist[-1])],
+                              "labels_s": round(t_labels, 2), "training_s": round(t_train, 2),
+                              "decision": "smallest order with cumulative probability >= 0.5",
+                              "label_rule": "global error budget 0.7e-6 spread by M2L cost (Lagrangian)"}}
+
+
+def cpu_baselines(args, x, q):
+    """The CPU oracle (C/OpenMP restatement of the reference CPU FMM, FP64)
+    on all host cores: median of cpu_reps solves after one warm-up, on the
+    headline workload and on the 65K 3D and 1M 2D configurations."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    cores = oracle.num_threads()
+    cn = args.cpu_n if args.cpu_n > 0 else args.n
+    if cn == args.n:  # the same particles
+        xs, qs = x, q
+    else:
+        rs = np.random.default_rng(args.seed + 7)
+        xs = rs.random((cn, 3))
+        qs = rs.uniform(-1.0, 1.0, cn)
+    cdepth = max(2, args.depth - int(round(np.log(args.n / cn) / np.log(8.0))))
+    tc, ts = median_time(lambda: oracle.fmm3d(xs, qs, args.terms, cdepth), args.cpu_reps)
+    log("cpu baseline %.1f s (median of %d)" % (tc, len(ts)))
+    head = {"value": cn / tc, "unit": "particles/s", "cores": cores, "kind": "port",
+            "sample": "CPU oracle (C/OpenMP restatement of the reference CPU FMM, FP64) on %d uniform particles, "
+                      "%d terms, depth %d: median of %d solves after a warm-up, %.2f s" %
+                      (cn, args.terms, cdepth, len(ts), tc)}
+    rng = np.random.default_rng(args.seed + 41)
+    x1, q1 = rng.random((65536, 3)), rng.uniform(-1.0, 1.0, 65536)
+    t1, _ = median_time(lambda: oracle.fmm3d(x1, q1, 9, 5), args.cpu_reps)
+    x2, q2 = rng.random((1048576, 2)), rng.uniform(-1.0, 1.0, 1048576)
+    t2, _ = median_time(lambda: oracle.fmm2d(x2, q2, 21, 9), args.cpu_reps)
+    log("cpu 65K p=8 %.3f s, 2D 1M p=20 %.3f s" % (t1, t2))
+    all_ = {"cores": cores, "protocol": "median of %d solves after one warm-up, all host cores" % args.cpu_reps,
+            "3d_65k_p8": {"seconds": round(t1, 4), "particles_per_s": 65536 / t1, "depth": 5},
+            "2d_1m_p20": {"seconds": round(t2, 4), "particles_per_s": 1048576 / t2, "depth": 9},
+            "3d_4m_p12": {"seconds": round(tc, 4), "particles_per_s": cn / tc, "depth": cdepth}}
+    return head, all_
 
 
 if __name__ == "__main__":

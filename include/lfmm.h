@@ -136,6 +136,15 @@ typedef struct lfmm_report {
                                     sums (self pair excluded)                 */
     double m2l_flops;            /* with LFMM_FLAG_COUNT_OPS: algorithmic FP64
                                     flops of all M2L operations at their orders */
+    /* With LFMM_FLAG_TIMINGS, summed device durations per kernel kind: P2M,
+     * M2M, the standalone L2L of the matrix-core levels, P2L (X lists), L2P
+     * (with the W lists' M2P) and the near-field pair step. */
+    double t_p2m_ms, t_m2m_ms, t_l2l_ms, t_p2l_ms, t_l2p_ms, t_p2p_ms;
+    int64_t p2p_source_reads;    /* with LFMM_FLAG_COUNT_OPS: particles of every
+                                    leaf's near list, summed over the leaves  */
+    double m2m_bytes, l2l_bytes; /* algorithmic HBM bytes of the M2M and the
+                                    standalone L2L launches (every expansion
+                                    read once, written once)                  */
 } lfmm_report;
 
 /* Create a context bound to a GPU (device index as HIP numbers them). */

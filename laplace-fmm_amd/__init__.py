@@ -61,7 +61,10 @@ class Report(ctypes.Structure):
                 ("t_leaf_kernel_ms", ctypes.c_double), ("t_m2l_mixed_ms", ctypes.c_double),
                 ("m2l_ops_mixed", ctypes.c_int64), ("num_boxes", ctypes.c_int64), ("xw_pairs", ctypes.c_int64),
                 ("terms_min", ctypes.c_int32), ("fallback", ctypes.c_int32), ("check_rel_l2", ctypes.c_double),
-                ("check_rel_l2_field", ctypes.c_double), ("m2l_flops", ctypes.c_double)]
+                ("check_rel_l2_field", ctypes.c_double), ("m2l_flops", ctypes.c_double),
+                ("t_p2m_ms", ctypes.c_double), ("t_m2m_ms", ctypes.c_double), ("t_l2l_ms", ctypes.c_double),
+                ("t_p2l_ms", ctypes.c_double), ("t_l2p_ms", ctypes.c_double), ("t_p2p_ms", ctypes.c_double),
+                ("p2p_source_reads", ctypes.c_int64), ("m2m_bytes", ctypes.c_double), ("l2l_bytes", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

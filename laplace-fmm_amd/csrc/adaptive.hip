@@ -264,13 +264,14 @@ __global__ void __launch_bounds__(256) k_ad_x_fill(const int32_t* __restrict__ x
                                                    const int32_t* __restrict__ leaf_begin, int g2,
                                                    int2* __restrict__ xrng, uint64_t* __restrict__ wk,
                                                    uint32_t* __restrict__ wv, uint32_t* __restrict__ wcnt,
-                                                   int32_t* __restrict__ xlist, const uint32_t* __restrict__ lex)
+                                                   int32_t* __restrict__ xlist, const uint32_t* __restrict__ lex,
+                                                   const int32_t* __restrict__ box2leaf)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nb) return;
     const uint32_t c = xcnt[t];
     if (c == 0) return;
-    xlist[lex[t]] = t;
+    if (box2leaf[g2 + t] < 0) xlist[lex[t]] = t;  // P2L for boxes with children only
     uint32_t w = xoff[t];
     for (uint32_t k = 0; k < c; ++k) {
         const int j = x27[(int64_t)t * 27 + k];
@@ -282,10 +283,70 @@ __global__ void __launch_bounds__(256) k_ad_x_fill(const int32_t* __restrict__ x
     }
 }
 
-__global__ void __launch_bounds__(256) k_ad_nonzero(const uint32_t* __restrict__ c, int64_t n, uint32_t* __restrict__ f)
+// boxes whose X list runs as P2L: those with children (a leaf takes its X
+// sources in its near list instead)
+__global__ void __launch_bounds__(256) k_ad_p2l_flag(const uint32_t* __restrict__ c, int64_t n,
+                                                     const int32_t* __restrict__ box2leaf, int g2,
+                                                     uint32_t* __restrict__ f)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i < n) f[i] = c[i] ? 1u : 0u;
+    if (i < n) f[i] = (c[i] && box2leaf[g2 + i] < 0) ? 1u : 0u;
+}
+
+// Near lists of the pair kernel: the U list, then the leaf's X sources (a
+// leaf target: direct sums, cheaper and exact), then the W boxes of at most
+// direct_max particles (direct sums instead of M2P).
+__device__ inline uint32_t ad_small_w(const int32_t* __restrict__ w_gid, uint32_t b, uint32_t e,
+                                      const int2* __restrict__ range, int direct_max)
+{
+    uint32_t c = 0;
+    for (uint32_t k = b; k < e; ++k) {
+        const int2 r = range[w_gid[k]];
+        c += (r.y - r.x <= direct_max) ? 1u : 0u;
+    }
+    return c;
+}
+
+__global__ void __launch_bounds__(256) k_ad_near_count(const int32_t* __restrict__ leaf_gid, int nleaf,
+                                                       const uint32_t* __restrict__ u_off,
+                                                       const uint32_t* __restrict__ x_off, int g2,
+                                                       const uint32_t* __restrict__ w_off,
+                                                       const int32_t* __restrict__ w_gid, const int2* __restrict__ range,
+                                                       int direct_max, uint32_t* __restrict__ ncnt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nleaf) return;
+    if (i == nleaf) {
+        ncnt[i] = 0u;
+        return;
+    }
+    const int g = leaf_gid[i];
+    uint32_t c = u_off[i + 1] - u_off[i];
+    if (x_off && g >= g2) c += x_off[g - g2 + 1] - x_off[g - g2];
+    if (w_gid) c += ad_small_w(w_gid, w_off[i], w_off[i + 1], range, direct_max);
+    ncnt[i] = c;
+}
+
+__global__ void __launch_bounds__(256) k_ad_near_fill(const int32_t* __restrict__ leaf_gid, int nleaf,
+                                                      const uint32_t* __restrict__ u_off, const int2* __restrict__ u_rng,
+                                                      const uint32_t* __restrict__ x_off, const int2* __restrict__ x_rng,
+                                                      int g2, const uint32_t* __restrict__ w_off,
+                                                      const int32_t* __restrict__ w_gid, const int2* __restrict__ range,
+                                                      int direct_max, const uint32_t* __restrict__ n_off,
+                                                      int2* __restrict__ n_rng)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nleaf) return;
+    const int g = leaf_gid[i];
+    uint32_t w = n_off[i];
+    for (uint32_t k = u_off[i]; k < u_off[i + 1]; ++k) n_rng[w++] = u_rng[k];
+    if (x_off && g >= g2)
+        for (uint32_t k = x_off[g - g2]; k < x_off[g - g2 + 1]; ++k) n_rng[w++] = x_rng[k];
+    if (w_gid)
+        for (uint32_t k = w_off[i]; k < w_off[i + 1]; ++k) {
+            const int2 r = range[w_gid[k]];
+            if (r.y - r.x <= direct_max) n_rng[w++] = r;
+        }
 }
 
 // sum over leaves of (particles) x (particles in its U list)
@@ -294,14 +355,19 @@ __global__ void __launch_bounds__(256) k_ad_pairs(const int32_t* __restrict__ le
                                                   unsigned long long* __restrict__ out)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    unsigned long long c = 0;
+    unsigned long long c = 0, rd = 0;
     if (i < nleaf) {
         unsigned long long t = 0;
         for (uint32_t k = u_off[i]; k < u_off[i + 1]; ++k) t += (unsigned long long)(u_rng[k].y - u_rng[k].x);
         c = t * (unsigned long long)(leaf_begin[i + 1] - leaf_begin[i]);
+        rd = t;
     }
-    for (int off = 32; off > 0; off >>= 1) c += (unsigned long long)__shfl_xor((long long)c, off);
+    for (int off = 32; off > 0; off >>= 1) {
+        c += (unsigned long long)__shfl_xor((long long)c, off);
+        rd += (unsigned long long)__shfl_xor((long long)rd, off);
+    }
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+    if ((threadIdx.x & 63) == 0 && rd) atomicAdd(out + 1, rd);
 }
 
 int bits_for(int64_t v)
@@ -510,7 +576,8 @@ int build_adaptive(lfmm_ctx* ctx, int leaf_max, Tree& T)
         LFMM_HIP(fill_u32(xcnt + nb, 1, 0u, st));
         int rc = scan_total(A, xcnt, T.x_off, nb, st);
         if (rc != LFMM_OK) return rc;
-        hipLaunchKernelGGL(k_ad_nonzero, dim3(nblk(nb + 1, 256)), dim3(256), 0, st, xcnt, (int64_t)nb + 1, nzf);
+        hipLaunchKernelGGL(k_ad_p2l_flag, dim3(nblk(nb, 256)), dim3(256), 0, st, xcnt, (int64_t)nb, T.box2leaf, g2, nzf);
+        LFMM_HIP(fill_u32(nzf + nb, 1, 0u, st));
         rc = scan_total(A, nzf, lex, nb, st);
         if (rc != LFMM_OK) return rc;
         // pair count and the xlist segments of every level
@@ -530,7 +597,7 @@ int build_adaptive(lfmm_ctx* ctx, int leaf_max, Tree& T)
         if (!T.x_rng || !T.xlist || !wk || !wv || !sws) return LFMM_ERR_NOMEM;
         LFMM_HIP(fill_u32(wcnt, nl + 1, 0u, st));
         hipLaunchKernelGGL(k_ad_x_fill, dim3(nblk(nb, 256)), dim3(256), 0, st, x27, xcnt, T.x_off, nb, T.leaf_begin, g2,
-                           T.x_rng, wk, wv, wcnt, T.xlist, lex);
+                           T.x_rng, wk, wv, wcnt, T.xlist, lex, T.box2leaf);
         rc = scan_total(A, wcnt, T.w_off, nl, st);
         if (rc != LFMM_OK) return rc;
         T.w_gid = (int32_t*)wv;
@@ -543,18 +610,41 @@ int build_adaptive(lfmm_ctx* ctx, int leaf_max, Tree& T)
     } else {
         LFMM_HIP(fill_u32(T.w_off, nl + 1, 0u, st));
     }
+
+    // ---- near lists of the pair kernel: U, plus the X and small W pairs
+    // that run as direct sums
+    {
+        const int g2 = depth > 2 ? T.loff[2] : total;
+        const uint32_t* xo = (depth > 2 && T.x_pairs > 0) ? T.x_off : nullptr;
+        const int32_t* wg = T.x_pairs > 0 ? T.w_gid : nullptr;
+        uint32_t* ncnt = take<uint32_t>(A, nl + 1);
+        T.n_off = take<uint32_t>(A, nl + 1);
+        if (!ncnt || !T.n_off) return LFMM_ERR_NOMEM;
+        hipLaunchKernelGGL(k_ad_near_count, dim3(nblk(nl + 1, 256)), dim3(256), 0, st, T.leaf_gid, (int)nl, T.u_off, xo,
+                           g2, T.w_off, wg, T.range_all, kDirectMax, ncnt);
+        int rc = scan_total(A, ncnt, T.n_off, nl, st);
+        if (rc != LFMM_OK) return rc;
+        uint32_t nt = 0;
+        if ((rc = read_u32(T.n_off + nl, &nt, st)) != LFMM_OK) return rc;
+        T.n_total = nt;
+        T.n_rng = take<int2>(A, nt);
+        if (!T.n_rng) return LFMM_ERR_NOMEM;
+        hipLaunchKernelGGL(k_ad_near_fill, dim3(nblk(nl, 256)), dim3(256), 0, st, T.leaf_gid, (int)nl, T.u_off, T.u_rng,
+                           xo, T.x_rng, g2, T.w_off, wg, T.range_all, kDirectMax, T.n_off, T.n_rng);
+    }
     return LFMM_OK;
 }
 
 int ad_count_pairs(const int32_t* leaf_begin, const uint32_t* u_off, const int2* u_rng, int64_t nleaf,
-                   unsigned long long* ws, int64_t* pairs, hipStream_t st)
+                   unsigned long long* ws, int64_t* pairs, int64_t* reads, hipStream_t st)
 {
-    LFMM_HIP(fill_u32((uint32_t*)ws, 2, 0u, st));
+    LFMM_HIP(fill_u32((uint32_t*)ws, 4, 0u, st));
     hipLaunchKernelGGL(k_ad_pairs, dim3(nblk(nleaf, 256)), dim3(256), 0, st, leaf_begin, u_off, u_rng, nleaf, ws);
-    unsigned long long h = 0;
-    LFMM_HIP(hipMemcpyAsync(&h, ws, 8, hipMemcpyDeviceToHost, st));
+    unsigned long long h[2] = {0, 0};
+    LFMM_HIP(hipMemcpyAsync(h, ws, 16, hipMemcpyDeviceToHost, st));
     LFMM_HIP(hipStreamSynchronize(st));
-    *pairs = (int64_t)h;
+    *pairs = (int64_t)h[0];
+    if (reads) *reads = (int64_t)h[1];
     return LFMM_OK;
 }
 

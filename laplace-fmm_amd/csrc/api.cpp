@@ -433,8 +433,8 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         // the operator tables of every bucket's order, before any launch
         for (int l = kMixedFirstLevel; l < depth; ++l)
             for (int t = 1; t <= LFMM_MAX_TERMS_3D; ++t) {
-                const int key = (l - 1) * 32 + t;
-                if (O.seg[key + 1] == O.seg[key]) continue;
+                const int key = ((l - 1) * 32 + t) * 8;
+                if (O.seg[key + 8] == O.seg[key]) continue;
                 const DevTables* DB = nullptr;
                 if ((rc = get_tables(ctx, t, &DB)) != LFMM_OK) return rc;
                 if ((flags & LFMM_FLAG_MIXED) && (rc = get_m16(ctx, t, &DB)) != LFMM_OK) return rc;
@@ -453,7 +453,16 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     const bool far = depth >= 3;
     size_t nev = 0;
     int m2l_launches = 0, mixed_launches = 0;
-    std::vector<char> ev_mixed;  // per event pair: matrix-core M2L or not
+    // per-kernel timing (LFMM_FLAG_TIMINGS): event pairs on the solve's
+    // stream around each launch, summed per kind
+    enum { EV_M2L64, EV_M2L16, EV_L2L, EV_M2M, EV_P2M, EV_P2L, EV_L2P, EV_P2P, EV_KINDS };
+    std::vector<int> ev_kind;  // per event pair
+    auto ev_open = [&]() -> hipError_t { return timed ? hipEventRecord(ctx_event(ctx, nev++), st) : hipSuccess; };
+    auto ev_close = [&](int kind) -> hipError_t {
+        if (!timed) return hipSuccess;
+        ev_kind.push_back(kind);
+        return hipEventRecord(ctx_event(ctx, nev++), st);
+    };
     const bool mixed = far && (flags & LFMM_FLAG_MIXED) && depth - 1 >= kMixedFirstLevel;
     if (mixed && tables) return LFMM_ERR_UNSUPPORTED;  // caller-placed tables hold no matrix-core operators
     uint4* xp = nullptr;
@@ -504,7 +513,9 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             pa.leaf_lvl = T.leaf_lvl;
             pa.absmax = mixed ? xmax : nullptr;
         }
+        LFMM_HIP(ev_open());
         LFMM_HIP(launch_p2m(N, pa, st));
+        LFMM_HIP(ev_close(EV_P2M));
         for (int l = leaf_level - 1; l >= 2; --l) {
             RtrLevel L{};
             L.level = l;
@@ -514,7 +525,9 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             L.src = outer[l + 1];
             L.dst = outer[l];
             if (mixed && l >= kMixedFirstLevel) L.absmax = xmax + l;
+            LFMM_HIP(ev_open());
             LFMM_HIP(launch_rtr(N, 0, DT->rt, L, st));
+            LFMM_HIP(ev_close(EV_M2M));
         }
     }
     double t_up = 0.0;
@@ -543,16 +556,22 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             if (bucketed) {
                 // one launch per order among the level's parents with children (parent lists)
                 for (int t = 1; t <= LFMM_MAX_TERMS_3D; ++t) {
-                    const int key = (l - 1) * 32 + t;
-                    const int64_t b0 = O.seg[key], b1 = O.seg[key + 1];
+                    const int key = ((l - 1) * 32 + t) * 8;
+                    const int64_t b0 = O.seg[key], b1 = O.seg[key + 8];
                     if (b1 == b0) continue;
                     const DevTables* DB = &ctx->tables.at(t);
+                    // the bucket's parents, one per child, sorted by the child's class
                     RtrLevel Lb = L;
                     Lb.plist = O.plist + b0;
                     Lb.num_parents = (int)(b1 - b0);
                     Lb.stride = NX;
                     Lb.dsplit = 1;
-                    Lb.tiles = (int)((b1 - b0 + rtr_tile(t) - 1) / rtr_tile(t));
+                    Lb.csorted = 1;
+                    for (int c = 0; c <= 8; ++c) Lb.coff[c] = (int)(O.seg[key + c] - b0);
+                    Lb.ctile[0] = 0;
+                    for (int c = 0; c < 8; ++c)
+                        Lb.ctile[c + 1] = Lb.ctile[c] + (Lb.coff[c + 1] - Lb.coff[c] + rtr_tile(t) - 1) / rtr_tile(t);
+                    Lb.tiles = Lb.ctile[8];
                     if (mixed) {
                         M2L16Args m{};
                         m.kmat = DB->m16;
@@ -576,20 +595,25 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                         m.plist = Lb.plist;
                         m.plist_count = Lb.num_parents;
                         m.stride = NX;
-                        m.tiles = (int)((b1 - b0 + m16_targets(t) - 1) / m16_targets(t));
                         m.dsplit = 1;
+                        m.csorted = 1;
+                        for (int c = 0; c <= 8; ++c) m.coff[c] = Lb.coff[c];
+                        m.ctile[0] = 0;
+                        for (int c = 0; c < 8; ++c)
+                            m.ctile[c + 1] = m.ctile[c] + (m.coff[c + 1] - m.coff[c] + m16_targets(t) - 1) / m16_targets(t);
+                        m.tiles = m.ctile[8];
                         LFMM_HIP(launch_m2l16_pack(t, m, st));
+                        LFMM_HIP(ev_open());
                         LFMM_HIP(launch_rtr(t, 2, DB->rt, Lb, st));
-                        if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+                        LFMM_HIP(ev_close(EV_L2L));
+                        LFMM_HIP(ev_open());
                         LFMM_HIP(launch_m2l16(t, m, st));
-                        if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
-                        ev_mixed.push_back(1);
+                        LFMM_HIP(ev_close(EV_M2L16));
                         ++mixed_launches;
                     } else {
-                        if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+                        LFMM_HIP(ev_open());
                         LFMM_HIP(launch_rtr(t, 1, DB->rt, Lb, st));
-                        if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
-                        ev_mixed.push_back(0);
+                        LFMM_HIP(ev_close(EV_M2L64));
                     }
                     ++m2l_launches;
                 }
@@ -626,12 +650,13 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                     if (!m.partial) return LFMM_ERR_NOMEM;
                 }
                 LFMM_HIP(launch_m2l16_pack(N, m, st));
+                LFMM_HIP(ev_open());
                 LFMM_HIP(launch_rtr(N, 2, DT->rt, L, st));
-                if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+                LFMM_HIP(ev_close(EV_L2L));
+                LFMM_HIP(ev_open());
                 LFMM_HIP(launch_m2l16(N, m, st));
                 if (m.dsplit > 1) LFMM_HIP(launch_sum_parts(m.partial, m.dsplit, cnt[l] * NX, inner[l], true, st));
-                if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
-                ev_mixed.push_back(1);
+                LFMM_HIP(ev_close(EV_M2L16));
                 ++mixed_launches;
             } else {
                 L.dsplit = dsplit_for((int64_t)L.tiles * 8);
@@ -639,11 +664,10 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                     L.partial = take<double2>(A, (int64_t)L.dsplit * cnt[l] * NX);
                     if (!L.partial) return LFMM_ERR_NOMEM;
                 }
-                if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+                LFMM_HIP(ev_open());
                 LFMM_HIP(launch_rtr(N, 1, DT->rt, L, st));
                 if (L.dsplit > 1) LFMM_HIP(launch_sum_parts(L.partial, L.dsplit, cnt[l] * NX, inner[l], false, st));
-                if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
-                ev_mixed.push_back(0);
+                LFMM_HIP(ev_close(EV_M2L64));
             }
             if (!bucketed) ++m2l_launches;
             if (adaptive && T.xseg[l + 1] > T.xseg[l]) {
@@ -653,7 +677,9 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                 px.x_off = T.x_off;
                 px.x_rng = T.x_rng;
                 px.g2 = T.loff[2];
+                LFMM_HIP(ev_open());
                 LFMM_HIP(launch_p2l(N, px, st));
+                LFMM_HIP(ev_close(EV_P2L));
             }
         }
     }
@@ -691,8 +717,10 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         la.leaf_key = T.leaf_key;
         la.leaf_begin = T.leaf_begin;
         la.local = inner_all;
-        la.nbr = T.u_rng;
-        la.nbr_off = T.u_off;
+        la.nbr = T.n_rng;
+        la.nbr_off = T.n_off;
+        la.box_range = T.range_all;
+        la.direct_max = kDirectMax;
         la.leaf_lvl = T.leaf_lvl;
         la.leaf_gid = T.leaf_gid;
         la.w_off = T.w_off;
@@ -719,11 +747,14 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     int32_t* p2p_ws = take<int32_t>(A, p2p_ws_elems(n, nleaf));
     uint32_t* p2p_scan = take<uint32_t>(A, scan_ws_elems(nleaf));
     if (!p2p_ws || !p2p_scan) return LFMM_ERR_NOMEM;
-    const size_t ev_leaf = nev;
-    if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
-    if (far) LFMM_HIP(launch_l2p(N, la, st));
+    if (far) {
+        LFMM_HIP(ev_open());
+        LFMM_HIP(launch_l2p(N, la, st));
+        LFMM_HIP(ev_close(EV_L2P));
+    }
+    LFMM_HIP(ev_open());
     LFMM_HIP(launch_p2p(la, p2p_ws, p2p_scan, st));  // near field, several targets per lane
-    if (timed) LFMM_HIP(hipEventRecord(ctx_event(ctx, nev++), st));
+    LFMM_HIP(ev_close(EV_P2P));
     // learned orders: the sampled direct-sum check judges the run
     double chk_p = 0.0, chk_f = 0.0;
     if (learned) {
@@ -757,18 +788,30 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         rep->check_rel_l2_field = chk_f;
         rep->m2l_flops = -1.0;
         if (timed) {
-            double tm = 0.0, tx = 0.0;
-            for (size_t i = 0; i + 1 < ev_leaf; i += 2) {
+            double tk[EV_KINDS] = {};
+            for (size_t k = 0; k < ev_kind.size(); ++k) {
                 float ms = 0.f;
-                LFMM_HIP(hipEventElapsedTime(&ms, ctx->events[i], ctx->events[i + 1]));
-                tm += ms;
-                if (ev_mixed[i / 2]) tx += ms;
+                LFMM_HIP(hipEventElapsedTime(&ms, ctx->events[2 * k], ctx->events[2 * k + 1]));
+                tk[ev_kind[k]] += ms;
             }
-            rep->t_m2l_mixed_ms = tx;
-            float ml = 0.f;
-            LFMM_HIP(hipEventElapsedTime(&ml, ctx->events[ev_leaf], ctx->events[ev_leaf + 1]));
-            rep->t_m2l_kernels_ms = tm;
-            rep->t_leaf_kernel_ms = ml;
+            rep->t_m2l_mixed_ms = tk[EV_M2L16];
+            rep->t_m2l_kernels_ms = tk[EV_M2L64] + tk[EV_M2L16];
+            rep->t_leaf_kernel_ms = tk[EV_L2P] + tk[EV_P2P];
+            rep->t_p2m_ms = tk[EV_P2M];
+            rep->t_m2m_ms = tk[EV_M2M];
+            rep->t_l2l_ms = tk[EV_L2L];
+            rep->t_p2l_ms = tk[EV_P2L];
+            rep->t_l2p_ms = tk[EV_L2P];
+            rep->t_p2p_ms = tk[EV_P2P];
+        }
+        // algorithmic HBM bytes of the M2M and standalone L2L launches: every
+        // expansion read once and written once (16 bytes per coefficient)
+        rep->m2m_bytes = 0.0;
+        rep->l2l_bytes = 0.0;
+        if (far) {
+            for (int l = 2; l + 1 < depth; ++l) rep->m2m_bytes += 16.0 * NX * (double)(cnt[l + 1] + cnt[l]);
+            if (mixed)
+                for (int l = kMixedFirstLevel; l < depth; ++l) rep->l2l_bytes += 16.0 * NX * (double)(cnt[l - 1] + cnt[l]);
         }
         rep->t_tree_ms = t_tree;
         rep->t_upward_ms = t_up;
@@ -779,9 +822,11 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             unsigned long long* cws = take<unsigned long long>(A, 2 + depth);
             if (!cws) return LFMM_ERR_NOMEM;
             std::vector<int64_t> per(depth, 0);
-            LFMM_HIP(count_ops(3, lkeys, lbegin, lut, cnt, cws, &rep->m2l_ops, &rep->p2p_pairs, per.data(), st));
+            LFMM_HIP(count_ops(3, lkeys, lbegin, lut, cnt, cws, &rep->m2l_ops, &rep->p2p_pairs, per.data(), st,
+                               &rep->p2p_source_reads));
             if (adaptive) {
-                rc = ad_count_pairs(T.leaf_begin, T.u_off, T.u_rng, T.nleaf, cws, &rep->p2p_pairs, st);
+                rc = ad_count_pairs(T.leaf_begin, T.n_off, T.n_rng, T.nleaf, cws, &rep->p2p_pairs,
+                                    &rep->p2p_source_reads, st);
                 if (rc != LFMM_OK) return rc;
             }
             rep->m2l_ops_mixed = 0;

@@ -167,6 +167,9 @@ struct Tree {
     int64_t x_pairs = 0;
     uint32_t* w_off = nullptr;        // W lists: [nleaf + 1] offsets into w_gid
     int32_t* w_gid = nullptr;         //   boxes whose multipoles act on the leaf
+    uint32_t* n_off = nullptr;        // near lists of the pair kernel: [nleaf + 1] offsets into n_rng
+    int2* n_rng = nullptr;            //   U, the leaf's X sources, W boxes of <= kDirectMax particles
+    int64_t n_total = 0;
     int total() const { return loff.empty() ? 0 : loff.back(); }
     AdLevels view() const;
 };
@@ -190,14 +193,20 @@ int upload_view(lfmm_ctx* ctx, const lfmm_points3d* S, int dim, DevPoints& P);
 int stage_inputs(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, int memory, int dim, DevPoints& P);
 int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, bool partition, Tree& T,
                bool keep_parents = false);
+// W-list boxes of at most this many particles act on a leaf by direct sums
+// instead of M2P (a direct sum costs ~20 flops per source and target, M2P
+// ~20 per coefficient, tri(13) = 91 at 13 terms); X lists into a leaf always
+// run as direct sums, P2L serves boxes with children only.
+constexpr int kDirectMax = 96;
 // Turns the uniform tree T (built with keep_parents) into the adaptive octree
 // of leaf size leaf_max and builds its U, X and W lists (adaptive.hip).
 int build_adaptive(lfmm_ctx* ctx, int leaf_max, Tree& T);
 // ---- per-box orders (order.hip)
 struct BoxOrders {
     uint32_t* box_terms = nullptr;   // [total] terms of each box (a parent: the largest of its children)
-    int32_t* plist = nullptr;        // parents with children sorted by (level, terms), indices at their level
-    std::vector<int64_t> seg;        // plist entries of key level*32 + terms: [seg[key], seg[key+1])
+    int32_t* plist = nullptr;        // per box of the bucketed levels, its parent (index at its level),
+                                     // sorted by (parent level, parent terms, child class)
+    std::vector<int64_t> seg;        // plist entries of key (level*32 + terms)*8 + class: [seg[key], seg[key+1])
     int max_terms = 0;               // over the leaves
     int min_leaf_terms = 0;
 };
@@ -227,9 +236,10 @@ int order_check(lfmm_ctx* ctx, const DevPoints& DP, const double4* src, int64_t 
                 const double* pot, const double* field, bool fmm_has_self, int samples, double* rel_pot,
                 double* rel_field);
 
-// Near-field pair count of an adaptive tree's U lists (ws: 1 counter).  Synchronises.
+// Near-field pair count of an adaptive tree's U lists and the particles of
+// the lists summed over the leaves (ws: 2 counters).  Synchronises.
 int ad_count_pairs(const int32_t* leaf_begin, const uint32_t* u_off, const int2* u_rng, int64_t nleaf,
-                   unsigned long long* ws, int64_t* pairs, hipStream_t st);
+                   unsigned long long* ws, int64_t* pairs, int64_t* reads, hipStream_t st);
 // The 3D solve behind lfmm_fmm3d (arguments checked, ctx->mu held, device
 // set).  tables: operator tables already on the device, or null to take the
 // context's cached ones (built on first use).

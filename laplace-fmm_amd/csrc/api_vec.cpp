@@ -114,6 +114,12 @@ int lfmm_fmm3d_vec(lfmm_ctx* c, const lfmm_points3d* sources, const lfmm_points3
                 part.t_m2l_kernels_ms += r.t_m2l_kernels_ms;
                 part.t_leaf_kernel_ms += r.t_leaf_kernel_ms;
                 part.t_m2l_mixed_ms += r.t_m2l_mixed_ms;
+                part.t_p2m_ms += r.t_p2m_ms;
+                part.t_m2m_ms += r.t_m2m_ms;
+                part.t_l2l_ms += r.t_l2l_ms;
+                part.t_p2l_ms += r.t_p2l_ms;
+                part.t_l2p_ms += r.t_l2p_ms;
+                part.t_p2p_ms += r.t_p2p_ms;
             }
         }
         return rc;

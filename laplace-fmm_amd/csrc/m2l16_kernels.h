@@ -432,9 +432,22 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int blk = xcd_local_block(blockIdx.x, gridDim.x);
-    const int tile = blk / (8 * a.dsplit);
-    const int part = (blk - tile * 8 * a.dsplit) >> 3;
-    const int c = blk & 7;
+    int tile, part, c;
+    int pbeg = 0, pend = a.plist ? a.plist_count : a.num_parents;  // this block's parent-list range
+    if (a.csorted) {
+        // class-sorted list: every column's parent has a class-c child;
+        // tile-major blocks with the 8 classes adjacent (see k_rtr)
+        c = blk & 7;
+        tile = blk >> 3;
+        if (tile >= a.ctile[c + 1] - a.ctile[c]) return;  // the whole block
+        part = 0;
+        pbeg = a.coff[c];
+        pend = a.coff[c + 1];
+    } else {
+        tile = blk / (8 * a.dsplit);
+        part = (blk - tile * 8 * a.dsplit) >> 3;
+        c = blk & 7;
+    }
     const int d_beg = part * kDirsPerClass / a.dsplit, d_end = (part + 1) * kDirsPerClass / a.dsplit;
     const int side = 1 << a.level;
 
@@ -444,9 +457,9 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
     uint32_t tx[CTW], ty[CTW], tz[CTW];
 #pragma unroll
     for (int ct = 0; ct < CTW; ++ct) {
-        const int Pi = tile * TC + 32 * (CTW * w + ct) + (lane & 31);
+        const int Pi = pbeg + tile * TC + 32 * (CTW * w + ct) + (lane & 31);
         int t = -1;
-        if (Pi < (a.plist ? a.plist_count : a.num_parents)) {
+        if (Pi < pend) {
             const int P = a.plist ? a.plist[Pi] : Pi;
             const int cb = a.child_begin[P], ce = a.child_begin[P + 1];
             for (int i = cb; i < ce; ++i)
@@ -715,6 +728,19 @@ hipError_t do_m2l16(const M2L16Args& a, hipStream_t st)
     static_assert(m16_targets(N) == (m16_rowfull(N) ? 128 * m16r_ctw(N) : 32 * m16_ct(N)), "tile width");
     const int targets = a.plist ? a.plist_count : a.num_parents;
     const int tiles = (targets + m16_targets(N) - 1) / m16_targets(N);
+    if (a.csorted) {
+        if (!a.plist || a.dsplit != 1 || !m16_rowfull(N) || a.coff[8] != a.plist_count) return hipErrorInvalidValue;
+        for (int c = 0; c < 8; ++c)
+            if (a.ctile[c + 1] - a.ctile[c] != (a.coff[c + 1] - a.coff[c] + m16_targets(N) - 1) / m16_targets(N))
+                return hipErrorInvalidValue;
+        int mt = 0;
+        for (int c = 0; c < 8; ++c) mt = mt > a.ctile[c + 1] - a.ctile[c] ? mt : a.ctile[c + 1] - a.ctile[c];
+        if constexpr (m16_rowfull(N))
+            if (mt > 0)
+                hipLaunchKernelGGL((k_m2l16r<N, m16r_ctw(N)>), dim3(8 * mt), dim3(256), 0, st, a, a.kmat,
+                                   (const uint4*)a.xp, a.class_dirs, a.dir_vec, a.kpre);
+        return hipGetLastError();
+    }
     if (tiles != a.tiles || a.dsplit < 1 || a.dsplit > kDirsPerClass) return hipErrorInvalidValue;
     if (a.plist && (a.dsplit != 1 || !m16_rowfull(N))) return hipErrorInvalidValue;
     if (tiles > 0) {

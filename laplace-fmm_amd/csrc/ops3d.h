@@ -70,6 +70,12 @@ struct RtrLevel {
     // (0: tri(N)); requires dsplit == 1
     const int32_t* plist = nullptr;
     int stride = 0;
+    // class-sorted parent list (M2L / L2L with plist): the parents with a
+    // class-c child are plist[coff[c] .. coff[c+1]), and blocks
+    // ctile[c] .. ctile[c+1] serve them (tiles of rtr_tile(N) parents)
+    int csorted = 0;
+    int coff[9] = {};
+    int ctile[9] = {};
 };
 
 struct LeafArgs {
@@ -105,6 +111,8 @@ struct LeafArgs {
     const int32_t* w_gid = nullptr;
     const uint64_t* keys_all = nullptr;
     const double2* outer_all = nullptr;
+    const int2* box_range = nullptr;   // particle range of every box (gid): W boxes of at most
+    int direct_max = 0;                // direct_max particles are in the near lists, not M2P
     int depth = 0;
     int loff[17] = {};
 };
@@ -139,6 +147,10 @@ struct M2L16Args {
     const int32_t* plist = nullptr;
     int plist_count = 0;
     int stride = 0;
+    // class-sorted plist, as RtrLevel (tiles of m16_targets(N) parents)
+    int csorted = 0;
+    int coff[9] = {};
+    int ctile[9] = {};
 };
 
 // Matrix-core M2L kernel per expansion size: the row-complete k_m2l16r

@@ -26,6 +26,7 @@
 #include "lfmm_internal.h"
 #include "ops3d.h"
 
+#include <algorithm>
 #include <type_traits>
 #include <utility>
 
@@ -366,20 +367,33 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
     // tile-major block order: the 8 child classes (and direction parts) of a
     // tile are adjacent, so the workgroups of one XCD read overlapping sources
     int tile, c, part = 0;
+    int pbeg = 0, pend = L.num_parents;  // this block's parent-list range
     if constexpr (KIND != KIND_M2M) {
-        const int per_tile = 8 * L.dsplit;
-        tile = blk / per_tile;
-        const int r = blk - tile * per_tile;
-        part = r >> 3;
-        c = r & 7;
+        if (L.csorted) {
+            // class-sorted list: every lane's parent has a class-c child.
+            // Blocks stay tile-major with the 8 classes adjacent (tile k of
+            // each class covers about the same region, so they share
+            // sources in L2); a class with fewer tiles leaves its blocks idle.
+            c = blk & 7;
+            tile = blk >> 3;
+            if (tile >= L.ctile[c + 1] - L.ctile[c]) return;  // the whole block
+            pbeg = L.coff[c];
+            pend = L.coff[c + 1];
+        } else {
+            const int per_tile = 8 * L.dsplit;
+            tile = blk / per_tile;
+            const int r = blk - tile * per_tile;
+            part = r >> 3;
+            c = r & 7;
+        }
     } else {
         tile = blk;
         c = 0;
     }
     // parent of this lane: the next of the level, or of the caller's list
     // (per-box orders: the parents whose children take this launch's N)
-    const int Pi = tile * TW + lane;
-    const bool pvalid = lok && Pi < L.num_parents;
+    const int Pi = pbeg + tile * TW + lane;
+    const bool pvalid = lok && Pi < pend;
     const int P = (pvalid && L.plist) ? L.plist[Pi] : Pi;
     // coefficient stride of the expansion arrays (larger than tri(N) when
     // the boxes of a level carry different orders)
@@ -705,6 +719,8 @@ __global__ void __launch_bounds__(256) k_l2p(LeafArgs a, const double2* __restri
         const uint32_t wb = a.w_off[leaf], we = a.w_off[leaf + 1];
         for (uint32_t k = wb; k < we; ++k) {
             const int g = a.w_gid[k];
+            const int2 br = a.box_range[g];
+            if (br.y - br.x <= a.direct_max) continue;  // summed directly by the pair kernel
             int lv = 0;
             while (lv + 1 < a.depth && g >= a.loff[lv + 1]) ++lv;
             const uint64_t sk = a.keys_all[g];
@@ -757,7 +773,23 @@ hipError_t do_rtr(int kind, const RtrTables& T, const RtrLevel& L, hipStream_t s
 {
     const int tiles = (L.num_parents + rtr_tile(N) - 1) / rtr_tile(N);
     if (tiles == 0) return hipSuccess;
-    if (L.tiles != tiles && kind != KIND_M2M) return hipErrorInvalidValue;
+    if (L.tiles != tiles && kind != KIND_M2M && !L.csorted) return hipErrorInvalidValue;
+    if (L.csorted && kind != KIND_M2M) {
+        if (L.dsplit != 1 || !L.plist || L.coff[8] != L.num_parents) return hipErrorInvalidValue;
+        for (int c = 0; c < 8; ++c)
+            if (L.ctile[c + 1] - L.ctile[c] != (L.coff[c + 1] - L.coff[c] + rtr_tile(N) - 1) / rtr_tile(N))
+                return hipErrorInvalidValue;
+        int mt = 0;
+        for (int c = 0; c < 8; ++c) mt = std::max(mt, L.ctile[c + 1] - L.ctile[c]);
+        if (mt == 0) return hipSuccess;
+        if (kind == KIND_L2L)
+            hipLaunchKernelGGL((k_rtr<N, KIND_L2L>), dim3(8 * mt), dim3(256), 0, st, T.rot, T.ephi, T.trans,
+                               T.dir_theta, T.dir_trans, T.dir_vec, T.class_dirs, L);
+        else
+            hipLaunchKernelGGL((k_rtr<N, KIND_M2L>), dim3(8 * mt), dim3(256), 0, st, T.rot, T.ephi, T.trans,
+                               T.dir_theta, T.dir_trans, T.dir_vec, T.class_dirs, L);
+        return hipGetLastError();
+    }
     if (kind == KIND_M2M)
         hipLaunchKernelGGL((k_rtr<N, KIND_M2M>), dim3(tiles), dim3(256), 0, st, T.rot, T.ephi, T.trans, T.dir_theta,
                            T.dir_trans, T.dir_vec, T.class_dirs, L);

@@ -357,10 +357,11 @@ __global__ void __launch_bounds__(256) k_box_terms(const int32_t* __restrict__ b
     box_terms[g] = t;
 }
 
-// launch buckets: parents with children at levels first_parent .. depth-2,
-// key (level, order), value the parent's index at its level
-__global__ void __launch_bounds__(256) k_bucket_keys(const int32_t* __restrict__ box2leaf,
-                                                     const uint32_t* __restrict__ box_terms, int g0, int g1, AdLevels A,
+// launch buckets: every box g0 .. g1-1 (levels >= the first bucketed level)
+// under its parent: key ((parent level) * 32 + parent terms) * 8 + child
+// class, value the parent's index at its level.  A (level, terms) bucket is
+// then its parents sorted by the class of the child, one entry per child.
+__global__ void __launch_bounds__(256) k_bucket_keys(const uint32_t* __restrict__ box_terms, int g0, int g1, AdLevels A,
                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                      uint32_t* __restrict__ hist)
 {
@@ -368,11 +369,12 @@ __global__ void __launch_bounds__(256) k_bucket_keys(const int32_t* __restrict__
     if (g >= g1) return;
     int l = 0;
     while (l + 1 < A.depth && g >= A.off[l + 1]) ++l;
-    const bool parent = box2leaf[g] < 0;
-    const uint32_t key = parent ? (uint32_t)l * 32u + box_terms[g] : 0xffffu;  // leaves sort last
+    const uint64_t k = A.keys[g];
+    const int p = od_find(A, l - 1, k >> 3);  // the parent is a kept box
+    const uint32_t key = ((uint32_t)(l - 1) * 32u + box_terms[A.off[l - 1] + p]) * 8u + (uint32_t)(k & 7u);
     keys[g - g0] = key;
-    vals[g - g0] = (uint32_t)(g - A.off[l]);
-    if (parent) atomicAdd(&hist[key], 1u);
+    vals[g - g0] = (uint32_t)p;
+    atomicAdd(&hist[key], 1u);
 }
 
 // algorithmic M2L flops of one level's targets at their orders: per box,
@@ -535,48 +537,48 @@ int order_box_terms(lfmm_ctx* ctx, const Tree& T, const uint32_t* leaf_terms, in
         hipLaunchKernelGGL(k_box_terms, dim3(nblk(cnt, 256)), dim3(256), 0, st, T.box2leaf, leaf_terms, T.lbegin[l],
                            T.loff[l], cnt, gnext, O.box_terms);
     }
-    // buckets of the parents of levels >= first_bucket_level
+    // buckets of the boxes of levels >= first_bucket_level, by parent
     const int depth = T.depth;
-    O.seg.assign((size_t)depth * 32 + 1, 0);
+    const int nkeys = 256 * depth;
+    O.seg.assign((size_t)nkeys + 1, 0);
     O.max_terms = 0;
     O.min_leaf_terms = 0;
-    const int pl0 = first_bucket_level - 1, pl1 = depth - 2;  // parent levels
-    std::vector<uint32_t> th(32 * depth + 2, 0);
-    uint32_t* hist = take<uint32_t>(A, 32 * depth + 2);
+    std::vector<uint32_t> th(nkeys, 0);
+    uint32_t* hist = take<uint32_t>(A, nkeys);
     if (!hist) return LFMM_ERR_NOMEM;
-    LFMM_HIP(fill_u32(hist, 32 * depth + 2, 0u, st));
+    LFMM_HIP(fill_u32(hist, nkeys, 0u, st));
     // largest and smallest leaf order
     uint32_t* mm = take<uint32_t>(A, 2);
     if (!mm) return LFMM_ERR_NOMEM;
     LFMM_HIP(fill_u32(mm, 1, 0xffffffffu, st));
     LFMM_HIP(fill_u32(mm + 1, 1, 0u, st));
     hipLaunchKernelGGL(k_terms_minmax, dim3(nblk(T.nleaf, 256)), dim3(256), 0, st, leaf_terms, T.nleaf, mm);
-    if (pl0 <= pl1) {
-        const int g0 = T.loff[pl0], g1 = T.loff[pl1 + 1];
+    if (first_bucket_level < depth && first_bucket_level >= 1) {
+        const int g0 = T.loff[first_bucket_level], g1 = T.loff[depth];
         const int64_t m = g1 - g0;
         uint64_t* keys = take<uint64_t>(A, m);
         uint32_t* vals = take<uint32_t>(A, m);
         void* ws = A.alloc((size_t)sort_ws_bytes(m));
         if (!keys || !vals || !ws) return LFMM_ERR_NOMEM;
-        hipLaunchKernelGGL(k_bucket_keys, dim3(nblk(m, 256)), dim3(256), 0, st, T.box2leaf, O.box_terms, g0, g1, T.view(),
-                           keys, vals, hist);
+        hipLaunchKernelGGL(k_bucket_keys, dim3(nblk(m, 256)), dim3(256), 0, st, O.box_terms, g0, g1, T.view(), keys,
+                           vals, hist);
         uint64_t* sk = keys;
         uint32_t* sv = vals;
         LFMM_HIP(radix_sort(keys, vals, m, 16, ws, st, &sk, &sv));
         O.plist = (int32_t*)sv;
     }
     uint32_t hm[2] = {0, 0};
-    LFMM_HIP(hipMemcpyAsync(th.data(), hist, sizeof(uint32_t) * 32 * depth, hipMemcpyDeviceToHost, st));
+    LFMM_HIP(hipMemcpyAsync(th.data(), hist, sizeof(uint32_t) * nkeys, hipMemcpyDeviceToHost, st));
     LFMM_HIP(hipMemcpyAsync(hm, mm, sizeof(hm), hipMemcpyDeviceToHost, st));
     LFMM_HIP(hipStreamSynchronize(st));
     O.min_leaf_terms = (int)hm[0];
     O.max_terms = (int)hm[1];
     int64_t run = 0;
-    for (int k = 0; k < 32 * depth; ++k) {
+    for (int k = 0; k < nkeys; ++k) {
         O.seg[k] = run;
         run += th[k];
     }
-    O.seg[32 * depth] = run;
+    O.seg[nkeys] = run;
     return LFMM_OK;
 }
 

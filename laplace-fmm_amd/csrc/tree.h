@@ -38,10 +38,13 @@ hipError_t run_compact(const uint64_t* keys, int64_t m, int sh, uint32_t* flags,
                        uint64_t* ukeys, int32_t* begin, int32_t* owner, int64_t* count_out, hipStream_t st);
 hipError_t build_lut(int dim, const uint64_t* ukeys, int64_t count, int level, int32_t* lut, hipStream_t st);
 // Exact M2L and near-field pair counts of a built tree (report only); ws
-// holds 2 + depth counters, m2l_per_level (optional) depth entries.  Synchronises.
+// holds 2 + depth counters, m2l_per_level (optional) depth entries;
+// p2p_reads (optional): the particles of every leaf's near list, summed over
+// the leaves.  Synchronises.
 hipError_t count_ops(int dim, const std::vector<uint64_t*>& keys, const std::vector<int32_t*>& begin,
                      const std::vector<int32_t*>& lut, const std::vector<int64_t>& cnt, unsigned long long* ws,
-                     int64_t* m2l_ops, int64_t* p2p_pairs, int64_t* m2l_per_level, hipStream_t st);
+                     int64_t* m2l_ops, int64_t* p2p_pairs, int64_t* m2l_per_level, hipStream_t st,
+                     int64_t* p2p_reads = nullptr);
 // p[0..n) = v with a fill kernel (safe inside stream capture)
 hipError_t fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t max_run(const int32_t* begin, int64_t count, uint32_t* maxv_dev, uint32_t* out_host, hipStream_t st);

@@ -481,10 +481,11 @@ __global__ void __launch_bounds__(256) k_count_m2l(const uint64_t* __restrict__ 
 template <int D>
 __global__ void __launch_bounds__(256) k_count_p2p(const uint64_t* __restrict__ keys, const int32_t* __restrict__ begin,
                                                    int size, int level, const int32_t* __restrict__ lut,
-                                                   unsigned long long* __restrict__ out)
+                                                   unsigned long long* __restrict__ out,
+                                                   unsigned long long* __restrict__ reads)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long c = 0;
+    unsigned long long c = 0, rd = 0;
     if (i < size) {
         const uint64_t k = keys[i];
         const int x = D == 3 ? compact3(k >> 2) : compact2(k >> 1), y = D == 3 ? compact3(k >> 1) : compact2(k),
@@ -499,9 +500,14 @@ __global__ void __launch_bounds__(256) k_count_p2p(const uint64_t* __restrict__ 
             if (j >= 0) s += (unsigned long long)(begin[j + 1] - begin[j]);
         }
         c = s * (unsigned long long)(begin[i + 1] - begin[i]);
+        rd = s;
     }
-    for (int off = 32; off > 0; off >>= 1) c += (unsigned long long)__shfl_xor((long long)c, off);
+    for (int off = 32; off > 0; off >>= 1) {
+        c += (unsigned long long)__shfl_xor((long long)c, off);
+        rd += (unsigned long long)__shfl_xor((long long)rd, off);
+    }
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+    if ((threadIdx.x & 63) == 0 && rd) atomicAdd(reads, rd);
 }
 
 // ------------------------------------------------------------------ near-field ranges
@@ -680,7 +686,8 @@ hipError_t build_lut(int dim, const uint64_t* ukeys, int64_t count, int level, i
 
 hipError_t count_ops(int dim, const std::vector<uint64_t*>& keys, const std::vector<int32_t*>& begin,
                      const std::vector<int32_t*>& lut, const std::vector<int64_t>& cnt, unsigned long long* ws,
-                     int64_t* m2l_ops, int64_t* p2p_pairs, int64_t* m2l_per_level, hipStream_t st)
+                     int64_t* m2l_ops, int64_t* p2p_pairs, int64_t* m2l_per_level, hipStream_t st,
+                     int64_t* p2p_reads)
 {
     const int depth = (int)cnt.size();
     hipError_t e = hipMemsetAsync(ws, 0, (2 + depth) * sizeof(unsigned long long), st);
@@ -696,10 +703,10 @@ hipError_t count_ops(int dim, const std::vector<uint64_t*>& keys, const std::vec
     const int L = depth - 1;
     if (dim == 2)
         hipLaunchKernelGGL(k_count_p2p<2>, dim3(nblk(cnt[L], 256)), dim3(256), 0, st, keys[L], begin[L], (int)cnt[L], L,
-                           L >= 2 ? lut[L] : nullptr, ws + 1);
+                           L >= 2 ? lut[L] : nullptr, ws + 1, ws);
     else
         hipLaunchKernelGGL(k_count_p2p<3>, dim3(nblk(cnt[L], 256)), dim3(256), 0, st, keys[L], begin[L], (int)cnt[L], L,
-                           L >= 2 ? lut[L] : nullptr, ws + 1);
+                           L >= 2 ? lut[L] : nullptr, ws + 1, ws);
     std::vector<unsigned long long> h(2 + depth);
     e = hipMemcpyAsync(h.data(), ws, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -710,6 +717,7 @@ hipError_t count_ops(int dim, const std::vector<uint64_t*>& keys, const std::vec
     }
     *m2l_ops = total;
     *p2p_pairs = (int64_t)h[1];
+    if (p2p_reads) *p2p_reads = (int64_t)h[0];
     return e;
 }
 

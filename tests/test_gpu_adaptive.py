@@ -109,6 +109,10 @@ def test_adaptive_lists_match_definitions(lfmm, request, seed, n, leaf_max):
     _, _, r = lfmm.fmm3d(x, q, terms=4, leaf_max=leaf_max, adaptive=True, count_ops=True)
     assert r["m2l_ops"] == sum(len(v) for v in V.values())
     assert r["xw_pairs"] == sum(len(v) for v in X.values())
-    assert r["p2p_pairs"] == sum(T.count[b] * sum(T.count[c] for c in us) for b, us in U.items())
+    # the pair kernel's near lists: U, then X into a leaf and W boxes of at
+    # most 96 particles as direct sums (cheaper than P2L / M2P, and exact)
+    near = {b: sum(T.count[c] for c in us) + sum(T.count[c] for c in X[b]) +
+            sum(T.count[c] for c in W[b] if T.count[c] <= 96) for b, us in U.items()}
+    assert r["p2p_pairs"] == sum(T.count[b] * near[b] for b in U)
     print("n=%d leaf_max=%d: %d boxes, %d leaves, %d X pairs, %d V pairs" %
           (n, leaf_max, len(gb), len(leaf_o), r["xw_pairs"], r["m2l_ops"]), flush=True)
