@@ -7,6 +7,9 @@
 //     FastMultipole(particles, eps = 1e-8, N = FMM_AUTO, tree_depth = FMM_AUTO)
 //     FastMultipole(sources, targets, eps, N, tree_depth)
 //     std::vector<std::complex<double>> forces;  std::vector<double> potentials;
+//   FastMultipole(particles, fmm::Options{p, leaf_max}) and the (sources,
+//     targets, Options) twin: order, leaf bound, self-term switch and the
+//     FP32 M2L option (Options::adaptive does not apply in 2D).
 // The computation runs inside the constructor; failures throw
 // std::runtime_error with the library status.  The (sources, targets)
 // constructor ignores the targets' charges (the reference requires q = 0).
@@ -37,6 +40,18 @@ public:
         Solve(source_particles, &target_particles, eps, N, tree_depth);
     }
 
+    FastMultipole(const std::vector<particle2d>& particles, const Options& o)
+    {
+        Solve(particles, nullptr, o.eps, o.terms(), o.depth(), o.leaf_max, o.flags(true) & ~LFMM_FLAG_ADAPTIVE);
+    }
+
+    FastMultipole(const std::vector<particle2d>& source_particles, const std::vector<particle2d>& target_particles,
+                  const Options& o)
+    {
+        Solve(source_particles, &target_particles, o.eps, o.terms(), o.depth(), o.leaf_max,
+              o.flags(false) & ~LFMM_FLAG_ADAPTIVE);
+    }
+
     std::vector<std::complex<double>> forces;
     std::vector<double> potentials;
 
@@ -55,7 +70,8 @@ private:
         return v;
     }
 
-    void Solve(const std::vector<particle2d>& src, const std::vector<particle2d>* tgt, double eps, int N, int depth)
+    void Solve(const std::vector<particle2d>& src, const std::vector<particle2d>* tgt, double eps, int N, int depth,
+               int leaf_max = 0, unsigned flags = 0u)
     {
         const lfmm_points2d s = view(src);
         lfmm_points2d t{};
@@ -63,7 +79,8 @@ private:
         const size_t n_out = tgt ? tgt->size() : src.size();
         potentials.assign(n_out, 0.0);
         forces.assign(n_out, std::complex<double>());
-        const int rc = lfmm_fmm2d(detail::thread_context(), &s, tgt ? &t : nullptr, eps, N, depth, 0, LFMM_MEM_HOST, 0u,
+        const int rc = lfmm_fmm2d(detail::thread_context(), &s, tgt ? &t : nullptr, eps, N, depth, leaf_max,
+                                  LFMM_MEM_HOST, flags,
                                   potentials.data(), forces.empty() ? nullptr : reinterpret_cast<double*>(forces.data()),
                                   nullptr);
         if (rc != LFMM_OK) throw std::runtime_error(std::string("lfmm_fmm2d: ") + lfmm_strerror(rc));

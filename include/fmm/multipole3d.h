@@ -12,6 +12,9 @@
 //   fmm::detail::InitMathConstants / cudaCopyMathConstants
 //     (src/common/special_functions.h:155-157): accepted and ignored; the
 //     library builds its own tables on first use.
+// Beyond the reference: constructors taking fmm::Options, i.e. the order p
+// and the leaf bound leaf_max directly ({p, leaf_max}), a self-term switch
+// (reference_conventions) and the adaptive-octree and matrix-core options.
 // The computation runs inside the constructor, as in the reference.  Where
 // the reference has no error path (unchecked CUDA calls), this header throws
 // std::runtime_error carrying the library status.  The (sources, targets)
@@ -66,6 +69,38 @@ struct particle {
 
 using particle3d = particle<point3d, double>;
 
+// Parameters of the (particles, Options) constructors.  The reference's
+// constructors take (eps, N, tree_depth) and pick the depth from the particle
+// count (src/cpu/multipole3d.cpp:1328-1337); these name the order and the
+// leaf bound instead: FastMultipole3d<double> f(particles, {p, leaf_max}).
+struct Options {
+    int p = FMM_AUTO;            // expansion order, N = p + 1 terms; FMM_AUTO: from eps, as the reference
+    int leaf_max = 64;           // the uniform depth (or adaptive split) keeps every leaf at <= leaf_max
+                                 // particles; <= 0: tree_depth or the reference's depth formula
+    bool reference_conventions = true;  // true: every potential includes the i == j pair at the clamped
+                                        // distance 1e-4, as the reference (src/common/simple_math.h:130-133);
+                                        // false: the self pair is skipped (single-set solves)
+    int tree_depth = FMM_AUTO;   // used when leaf_max <= 0
+    double eps = 1.e-8;          // used when p == FMM_AUTO
+    bool adaptive = false;       // 3D: adaptive octree (leaves on any level, U/V/W/X lists)
+    bool mixed = false;          // M2L of levels >= 4 on the matrix cores (split FP16 with FP32 sums)
+
+    Options() = default;
+    Options(int p_, int leaf_max_, bool reference_conventions_ = true)
+        : p(p_), leaf_max(leaf_max_), reference_conventions(reference_conventions_) {}
+
+    int terms() const { return p == FMM_AUTO ? FMM_AUTO : p + 1; }
+    int depth() const { return leaf_max > 0 ? FMM_AUTO : tree_depth; }
+    unsigned flags(bool single_set) const
+    {
+        unsigned f = 0;
+        if (!reference_conventions && single_set) f |= LFMM_FLAG_EXCLUDE_SELF;
+        if (adaptive) f |= LFMM_FLAG_ADAPTIVE;
+        if (mixed) f |= LFMM_FLAG_MIXED;
+        return f;
+    }
+};
+
 namespace detail {
 inline void InitMathConstants(int) {}
 inline void cudaCopyMathConstants() {}
@@ -109,6 +144,17 @@ public:
         Solve(source_particles, &target_particles, eps, N, tree_depth);
     }
 
+    FastMultipole3d(const std::vector<particle_t>& particles, const Options& o)
+    {
+        Solve(particles, nullptr, o.eps, o.terms(), o.depth(), o.leaf_max, o.flags(true));
+    }
+
+    FastMultipole3d(const std::vector<particle_t>& source_particles, const std::vector<particle_t>& target_particles,
+                    const Options& o)
+    {
+        Solve(source_particles, &target_particles, o.eps, o.terms(), o.depth(), o.leaf_max, o.flags(false));
+    }
+
     std::vector<Vector3d> forces;
     std::vector<double> potentials;
 
@@ -127,7 +173,8 @@ private:
         return v;
     }
 
-    void Solve(const std::vector<particle_t>& src, const std::vector<particle_t>* tgt, double eps, int N, int depth)
+    void Solve(const std::vector<particle_t>& src, const std::vector<particle_t>* tgt, double eps, int N, int depth,
+               int leaf_max = 0, unsigned flags = 0u)
     {
         const lfmm_points3d s = view(src);
         lfmm_points3d t{};
@@ -135,8 +182,9 @@ private:
         const size_t n_out = tgt ? tgt->size() : src.size();
         potentials.assign(n_out, 0.0);
         forces.assign(n_out, Vector3d());
-        const int rc = lfmm_fmm3d(detail::thread_context(), &s, tgt ? &t : nullptr, eps, N, depth, 0, LFMM_MEM_HOST, 0u,
-                                  potentials.data(), forces.empty() ? nullptr : &forces[0][0], nullptr);
+        const int rc = lfmm_fmm3d(detail::thread_context(), &s, tgt ? &t : nullptr, eps, N, depth, leaf_max,
+                                  LFMM_MEM_HOST, flags, potentials.data(), forces.empty() ? nullptr : &forces[0][0],
+                                  nullptr);
         if (rc != LFMM_OK) throw std::runtime_error(std::string("lfmm_fmm3d: ") + lfmm_strerror(rc));
     }
 };
@@ -161,6 +209,17 @@ public:
         Solve(source_particles, &target_particles, eps, N, tree_depth);
     }
 
+    FastMultipole3d(const std::vector<particle_t>& particles, const Options& o)
+    {
+        Solve(particles, nullptr, o.eps, o.terms(), o.depth(), o.leaf_max, o.flags(true));
+    }
+
+    FastMultipole3d(const std::vector<particle_t>& source_particles, const std::vector<particle_t>& target_particles,
+                    const Options& o)
+    {
+        Solve(source_particles, &target_particles, o.eps, o.terms(), o.depth(), o.leaf_max, o.flags(false));
+    }
+
     std::vector<Vector3d> forces;
     std::vector<double> potentials;
 
@@ -178,15 +237,16 @@ private:
         return v;
     }
 
-    void Solve(const std::vector<particle_t>& src, const std::vector<particle_t>* tgt, double eps, int N, int depth)
+    void Solve(const std::vector<particle_t>& src, const std::vector<particle_t>* tgt, double eps, int N, int depth,
+               int leaf_max = 0, unsigned flags = 0u)
     {
         const lfmm_points3d s = view(src);
         lfmm_points3d t{};
         if (tgt) t = view(*tgt);
         const size_t n_out = tgt ? tgt->size() : src.size();
         forces.assign(n_out, Vector3d());
-        const int rc = lfmm_fmm3d_vec(detail::thread_context(), &s, tgt ? &t : nullptr, eps, N, depth, 0,
-                                      LFMM_MEM_HOST, 0u, forces.empty() ? nullptr : &forces[0][0], nullptr);
+        const int rc = lfmm_fmm3d_vec(detail::thread_context(), &s, tgt ? &t : nullptr, eps, N, depth, leaf_max,
+                                      LFMM_MEM_HOST, flags, forces.empty() ? nullptr : &forces[0][0], nullptr);
         if (rc != LFMM_OK) throw std::runtime_error(std::string("lfmm_fmm3d_vec: ") + lfmm_strerror(rc));
     }
 };
@@ -199,6 +259,11 @@ FastMultipole3d(const std::vector<fmm::particle<point3d, value>>&, double = 1.e-
 template <typename value>
 FastMultipole3d(const std::vector<fmm::particle<point3d, value>>&, const std::vector<fmm::particle<point3d, value>>&,
                 double = 1.e-8, int = FMM_AUTO, int = FMM_AUTO) -> FastMultipole3d<value>;
+template <typename value>
+FastMultipole3d(const std::vector<fmm::particle<point3d, value>>&, const Options&) -> FastMultipole3d<value>;
+template <typename value>
+FastMultipole3d(const std::vector<fmm::particle<point3d, value>>&, const std::vector<fmm::particle<point3d, value>>&,
+                const Options&) -> FastMultipole3d<value>;
 
 }  // namespace gpu
 

@@ -538,6 +538,8 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     // direction parts: levels with fewer workgroups than the chip holds split
     // their 189 directions over more workgroups (deterministic partial sums)
     auto dsplit_for = [](int64_t wgs) { return (int)std::max<int64_t>(1, std::min<int64_t>(32, (1024 + wgs - 1) / wgs)); };
+    double2* list_partial = nullptr;  // direction parts of the matrix-core M2L buckets
+    int64_t list_partial_elems = 0;
     if (far) {
         for (int l = 2; l < depth; ++l) {
             RtrLevel L{};
@@ -602,12 +604,28 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                         for (int c = 0; c < 8; ++c)
                             m.ctile[c + 1] = m.ctile[c] + (m.coff[c + 1] - m.coff[c] + m16_targets(t) - 1) / m16_targets(t);
                         m.tiles = m.ctile[8];
+                        // a workgroup runs all 189 directions of its targets:
+                        // small buckets split the directions into parts so
+                        // that about 512 workgroups share the work
+                        m.dsplit = (int)std::max<int64_t>(1, std::min<int64_t>(32, (512 + m.tiles - 1) /
+                                                                                      std::max(1, m.tiles)));
+                        if (m.dsplit > 1) {
+                            // one buffer serves every bucket (the launches are ordered on the stream)
+                            const int64_t need = (int64_t)m.dsplit * m.plist_count * tri(t);
+                            if (need > list_partial_elems) {
+                                list_partial = take<double2>(A, need);
+                                if (!list_partial) return LFMM_ERR_NOMEM;
+                                list_partial_elems = need;
+                            }
+                            m.partial = list_partial;
+                        }
                         LFMM_HIP(launch_m2l16_pack(t, m, st));
                         LFMM_HIP(ev_open());
                         LFMM_HIP(launch_rtr(t, 2, DB->rt, Lb, st));
                         LFMM_HIP(ev_close(EV_L2L));
                         LFMM_HIP(ev_open());
                         LFMM_HIP(launch_m2l16(t, m, st));
+                        LFMM_HIP(launch_sum_list_parts(t, m, st));
                         LFMM_HIP(ev_close(EV_M2L16));
                         ++mixed_launches;
                     } else {

@@ -12,6 +12,7 @@
 #include "device_common.h"
 #include "direct.h"
 
+#include <algorithm>
 #include <mutex>
 
 namespace lfmm {
@@ -103,6 +104,62 @@ __global__ void __launch_bounds__(kTile) k_direct2d(const double4* __restrict__ 
     }
 }
 
+// Few targets, many sources: block (x, y) sums targets x * kTile .. over
+// source chunk y into part[y * nt + i] = (pot, field); k_direct3d_join adds
+// the chunks in order.  Spreads a sampled check (hundreds of targets against
+// millions of sources) over the whole GPU.
+__global__ void __launch_bounds__(kTile) k_direct3d_part(const double4* __restrict__ tgt, int64_t nt,
+                                                        const double4* __restrict__ src, int64_t ns, int64_t chunk,
+                                                        double4* __restrict__ part)
+{
+    __shared__ double4 tile[kTile];
+    const int64_t i = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    const double4 t = i < nt ? tgt[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const int64_t j_beg = (int64_t)blockIdx.y * chunk, j_end = min(ns, j_beg + chunk);
+    double p = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int64_t base = j_beg; base < j_end; base += kTile) {
+        __syncthreads();
+        const int64_t j0 = base + threadIdx.x;
+        tile[threadIdx.x] = j0 < j_end ? src[j0] : make_double4(0.0, 0.0, 0.0, 0.0);
+        __syncthreads();
+        const int cnt = (int)min<int64_t>(kTile, j_end - base);
+        for (int j = 0; j < cnt; ++j) {
+            const double4 s = tile[j];
+            const double dx = t.x - s.x, dy = t.y - s.y, dz = t.z - s.z;
+            const double r2 = fmax(fma(dx, dx, fma(dy, dy, dz * dz)), 1.e-8);
+            const double ir = rsqrt_full(r2);
+            const double qi = s.w * ir;
+            p += qi;
+            const double q3 = qi * ir * ir;
+            fx = fma(q3, dx, fx);
+            fy = fma(q3, dy, fy);
+            fz = fma(q3, dz, fz);
+        }
+    }
+    if (i < nt) part[(int64_t)blockIdx.y * nt + i] = make_double4(p, fx, fy, fz);
+}
+
+__global__ void __launch_bounds__(256) k_direct3d_join(const double4* __restrict__ part, int64_t nt, int splits,
+                                                      double* __restrict__ pot, double* __restrict__ field)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    double p = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int y = 0; y < splits; ++y) {
+        const double4 v = part[(int64_t)y * nt + i];
+        p += v.x;
+        fx += v.y;
+        fy += v.z;
+        fz += v.w;
+    }
+    if (pot) pot[i] = p;
+    if (field) {
+        field[3 * i] = fx;
+        field[3 * i + 1] = fy;
+        field[3 * i + 2] = fz;
+    }
+}
+
 // Field of vector charges from the fields of their components: with
 // E_c = sum_j q_j^c r / |r|^3, sum_j q_j x r / |r|^3 = (E_2.z - E_3.y,
 // E_3.x - E_1.z, E_1.y - E_2.x) (the reference's forces_cross,
@@ -141,6 +198,27 @@ hipError_t launch_direct(int dim, const double4* tgt, int64_t nt, const double4*
         hipLaunchKernelGGL(k_direct3d, grid, dim3(kTile), 0, st, tgt, nt, src, ns, exclude_self ? 1 : 0, pot, field);
     else
         hipLaunchKernelGGL(k_direct2d, grid, dim3(kTile), 0, st, tgt, nt, src, ns, exclude_self ? 1 : 0, pot, field);
+    return hipGetLastError();
+}
+
+int direct3d_split_count(int64_t nt, int64_t ns)
+{
+    if (nt <= 0 || ns <= 0) return 1;
+    const int64_t tblocks = (nt + kTile - 1) / kTile;
+    const int64_t want = (2048 + tblocks - 1) / tblocks;        // about 8 workgroups per CU
+    const int64_t most = (ns + 4 * kTile - 1) / (4 * kTile);    // at least 4 tiles per chunk
+    return (int)std::max<int64_t>(1, std::min<int64_t>({want, most, 65535}));
+}
+
+hipError_t launch_direct3d_split(const double4* tgt, int64_t nt, const double4* src, int64_t ns, double* pot,
+                                 double* field, double4* part, int splits, hipStream_t st)
+{
+    if (nt <= 0) return hipSuccess;
+    const int64_t chunk = ((ns + splits - 1) / splits + kTile - 1) / kTile * kTile;
+    const dim3 grid((unsigned)((nt + kTile - 1) / kTile), (unsigned)splits);
+    hipLaunchKernelGGL(k_direct3d_part, grid, dim3(kTile), 0, st, tgt, nt, src, ns, chunk, part);
+    hipLaunchKernelGGL(k_direct3d_join, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, part, nt, splits, pot,
+                       field);
     return hipGetLastError();
 }
 

@@ -12,7 +12,8 @@
 // split into fp16 hi + lo (22 significant bits).  Each product is
 //   hi*hi + hi*lo + lo*hi   (the lo*lo term, ~2^-22 relative, is dropped)
 // computed exactly on the matrix cores and accumulated in FP32 over all K and
-// all 189 directions.  From 12 terms, operator blocks (32 rows x 16 K) whose
+// all 189 directions (k_m2l16r: rows of degree n <= 5 per direction, then
+// summed over the directions).  From 12 terms, operator blocks (32 rows x 16 K) whose
 // largest entry is below 2^-11 of the largest operator entry use hi*hi alone,
 // and blocks below 2^-22 are skipped: either way the dropped terms are bounded
 // by 2^-22 max|K| max|X|, the absolute bound of the split itself.  The result
@@ -436,11 +437,13 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
     int pbeg = 0, pend = a.plist ? a.plist_count : a.num_parents;  // this block's parent-list range
     if (a.csorted) {
         // class-sorted list: every column's parent has a class-c child;
-        // tile-major blocks with the 8 classes adjacent (see k_rtr)
+        // tile-major blocks with the 8 classes adjacent (see k_rtr), and
+        // the direction parts of a tile next to each other
         c = blk & 7;
-        tile = blk >> 3;
+        const int rest = blk >> 3;
+        tile = rest / a.dsplit;
+        part = rest - tile * a.dsplit;
         if (tile >= a.ctile[c + 1] - a.ctile[c]) return;  // the whole block
-        part = 0;
         pbeg = a.coff[c];
         pend = a.coff[c + 1];
     } else {
@@ -453,11 +456,12 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
 
     // targets of this lane: the class-c child of parent tile * TC + column,
     // column 32 * (CTW * w + ct) + lane % 32
-    int tgt[CTW];
+    int tgt[CTW], pidx[CTW];
     uint32_t tx[CTW], ty[CTW], tz[CTW];
 #pragma unroll
     for (int ct = 0; ct < CTW; ++ct) {
         const int Pi = pbeg + tile * TC + 32 * (CTW * w + ct) + (lane & 31);
+        pidx[ct] = Pi;
         int t = -1;
         if (Pi < pend) {
             const int P = a.plist ? a.plist[Pi] : Pi;
@@ -568,6 +572,16 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
             for (int ct = 0; ct < CTW; ++ct) acc[r][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(z8, z8, z16, 0, 0, 0);
     }
 
+    // Row tile 0 (degrees n <= 5, which carry most of the potential inside
+    // the target box) restarts from zero every direction and its per-
+    // direction sums are added into tot: the FP32 rounding of each MFMA then
+    // scales with one direction's contribution, not with the running total
+    // over 189 directions (about 5x less error where one source box
+    // dominates, as in clustered inputs).
+    m16_acc tot[CTW];
+#pragma unroll
+    for (int ct = 0; ct < CTW; ++ct) tot[ct] = m16_acc{};
+
     int dcur = dir_of(d_beg);
     int bk3, bk1;
     prefixes(dcur, bk3, bk1);
@@ -668,7 +682,11 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
         bk3 = bk3n;
         bk1 = bk1n;
 #pragma unroll
-        for (int ct = 0; ct < CTW; ++ct) bx[ct] = bxn[ct];
+        for (int ct = 0; ct < CTW; ++ct) {
+            bx[ct] = bxn[ct];
+            tot[ct] += acc[0][ct];
+            acc[0][ct] = m16_acc{};
+        }
     }
 
     // ---- epilogue: L += acc / (scale of K * scale of X) in FP64, or the
@@ -676,12 +694,16 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
     // part's slice is written: the imaginary parts of the m = 0 coefficients,
     // which have no real coordinate, as zeros)
     const double inv = a.kinv / m16_xscale(*a.xmax);
-    double2* const dst = a.dsplit > 1 ? a.partial + (int64_t)part * a.size * tri(N) : a.dst;
+    // split directions: the part's slice of the level (indexed by target),
+    // or with a class-sorted list the part's slice of the list (indexed by
+    // list position)
+    const int64_t part_size = a.csorted ? a.plist_count : a.size;
+    double2* const dst = a.dsplit > 1 ? a.partial + (int64_t)part * part_size * tri(N) : a.dst;
     const int64_t SX = (a.dsplit == 1 && a.stride > 0) ? a.stride : tri(N);
 #pragma unroll
     for (int ct = 0; ct < CTW; ++ct) {
         if (tgt[ct] < 0) continue;
-        double* out = (double*)(dst + (int64_t)tgt[ct] * SX);
+        double* out = (double*)(dst + (int64_t)(a.dsplit > 1 && a.csorted ? pidx[ct] : tgt[ct]) * SX);
         if (a.dsplit > 1) {
             // lanes l and l + 32 hold the same target
             for (int n = lane >> 5; n < N; n += 2) out[2 * tri(n) + 1] = 0.0;
@@ -693,7 +715,7 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
                 const int row = rt * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
                 if (row < NR) {
                     const int o = m16_coef_offset(row);
-                    const double val = (double)acc[rt][ct][v] * inv;
+                    const double val = (double)(rt == 0 ? tot[ct][v] : acc[rt][ct][v]) * inv;
                     if (a.dsplit > 1) out[o] = val;
                     else out[o] += val;
                 }
@@ -729,7 +751,9 @@ hipError_t do_m2l16(const M2L16Args& a, hipStream_t st)
     const int targets = a.plist ? a.plist_count : a.num_parents;
     const int tiles = (targets + m16_targets(N) - 1) / m16_targets(N);
     if (a.csorted) {
-        if (!a.plist || a.dsplit != 1 || !m16_rowfull(N) || a.coff[8] != a.plist_count) return hipErrorInvalidValue;
+        if (!a.plist || a.dsplit < 1 || a.dsplit > kDirsPerClass || !m16_rowfull(N) || a.coff[8] != a.plist_count ||
+            (a.dsplit > 1 && !a.partial))
+            return hipErrorInvalidValue;
         for (int c = 0; c < 8; ++c)
             if (a.ctile[c + 1] - a.ctile[c] != (a.coff[c + 1] - a.coff[c] + m16_targets(N) - 1) / m16_targets(N))
                 return hipErrorInvalidValue;
@@ -737,7 +761,7 @@ hipError_t do_m2l16(const M2L16Args& a, hipStream_t st)
         for (int c = 0; c < 8; ++c) mt = mt > a.ctile[c + 1] - a.ctile[c] ? mt : a.ctile[c + 1] - a.ctile[c];
         if constexpr (m16_rowfull(N))
             if (mt > 0)
-                hipLaunchKernelGGL((k_m2l16r<N, m16r_ctw(N)>), dim3(8 * mt), dim3(256), 0, st, a, a.kmat,
+                hipLaunchKernelGGL((k_m2l16r<N, m16r_ctw(N)>), dim3(8 * mt * a.dsplit), dim3(256), 0, st, a, a.kmat,
                                    (const uint4*)a.xp, a.class_dirs, a.dir_vec, a.kpre);
         return hipGetLastError();
     }

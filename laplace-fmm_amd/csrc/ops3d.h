@@ -147,7 +147,10 @@ struct M2L16Args {
     const int32_t* plist = nullptr;
     int plist_count = 0;
     int stride = 0;
-    // class-sorted plist, as RtrLevel (tiles of m16_targets(N) parents)
+    // class-sorted plist, as RtrLevel (tiles of m16_targets(N) parents);
+    // with dsplit > 1 part p of list entry i goes to
+    // partial + (p * plist_count + i) * tri(N), added into dst by
+    // launch_sum_list_parts
     int csorted = 0;
     int coff[9] = {};
     int ctile[9] = {};
@@ -162,7 +165,7 @@ struct M2L16Args {
 #endif
 constexpr bool m16_rowfull(int N) { return N >= LFMM_M16R_MIN_N; }
 // k_m2l16r keeps two column tiles per wave up to this many row tiles of 32
-constexpr int m16r_max_rt2 = 7;
+constexpr int m16r_max_rt2 = 6;
 constexpr int m16_targets(int N)
 {
     return m16_rowfull(N) ? ((N * N + 31) / 32 <= m16r_max_rt2 ? 256 : 128) : (N <= 16 ? 64 : 32);
@@ -185,6 +188,9 @@ hipError_t launch_m2l16_pack(int N, const M2L16Args& a, hipStream_t st);
 hipError_t launch_m2l16(int N, const M2L16Args& a, hipStream_t st);
 // dst[i] = (add ? dst[i] : 0) + sum_p partial[p * count + i]   (m2l16.hip)
 hipError_t launch_sum_parts(const double2* partial, int parts, int64_t count, double2* dst, bool add, hipStream_t st);
+// the direction parts of a class-sorted list (M2L16Args::csorted,
+// dsplit > 1), added into the listed targets' locals (m2l16.hip)
+hipError_t launch_sum_list_parts(int N, const M2L16Args& a, hipStream_t st);
 
 
 }  // namespace lfmm

@@ -619,10 +619,12 @@ int order_check(lfmm_ctx* ctx, const DevPoints& DP, const double4* src, int64_t 
     double* dp = take<double>(A, S);
     double* df = take<double>(A, 3 * S);
     double4* out = take<double4>(A, S);
-    if (!rec || !at || !dp || !df || !out) return LFMM_ERR_NOMEM;
+    const int splits = direct3d_split_count(S, n);
+    double4* part = take<double4>(A, (int64_t)splits * S);
+    if (!rec || !at || !dp || !df || !out || !part) return LFMM_ERR_NOMEM;
     hipLaunchKernelGGL(k_check_gather, dim3(nblk(S, 256)), dim3(256), 0, st, DP.pos, DP.ps, DP.q, DP.qs, n_out, S,
                        has_q ? 1 : 0, rec, at);
-    LFMM_HIP(launch_direct(3, rec, S, src, n, false, dp, df, st));
+    LFMM_HIP(launch_direct3d_split(rec, S, src, n, dp, df, part, splits, st));
     hipLaunchKernelGGL(k_check_diff, dim3(nblk(S, 256)), dim3(256), 0, st, rec, at, S, dp, df, pot, field,
                        fmm_has_self ? 1 : 0, out);
     std::vector<double4> h(S);

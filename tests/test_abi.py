@@ -67,6 +67,15 @@ int main() {
         std::printf("%zu %zu\n", f.potentials.size(), f.forces.size());
     } catch (const std::exception& e) { std::printf("error: %s\n", e.what()); }
     try {
+        fmm::gpu::FastMultipole3d g(p, {8, 64});           // (p, leaf_max), deduced <double>
+        fmm::gpu::FastMultipole3d h(p, p, fmm::Options(8, 64, false));
+        fmm::Options o; o.adaptive = true; o.mixed = true;
+        fmm::gpu::FastMultipole3d<double> k(p, o);
+        std::printf("%zu %zu %zu\n", g.potentials.size(), h.potentials.size(), k.potentials.size());
+        fmm::gpu::FastMultipole f3(p2, {8, 64, false});
+        std::printf("%zu\n", f3.potentials.size());
+    } catch (const std::exception& e) { std::printf("error: %s\n", e.what()); }
+    try {
         fmm::gpu::FastMultipole f2(p2, 1e-8, 8, 3);
         std::printf("%zu %zu\n", f2.potentials.size(), f2.forces.size());
     } catch (const std::exception& e) { std::printf("error: %s\n", e.what()); }

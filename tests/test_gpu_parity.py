@@ -270,3 +270,49 @@ int main() {
     a = np.loadtxt(out.stdout.splitlines())
     pot, field, _ = lfmm.fmm3d(a[:, :3], a[:, 3], terms=9, tree_depth=4)
     assert np.array_equal(pot, a[:, 4]) and np.array_equal(field, a[:, 5:8])
+
+
+def test_cpp_options_overload_matches_python(lfmm, tmp_path, request):
+    """The (particles, {p, leaf_max, reference_conventions}) constructors
+    give the library's own results for terms p + 1, that leaf bound and the
+    self-term switch, on the uniform and the adaptive tree."""
+    case_name(request)
+    src = tmp_path / "use.cpp"
+    src.write_text(r'''
+#include "fmm/multipole3d.h"
+#include <cstdio>
+#include <random>
+int main() {
+    std::mt19937_64 g(9);
+    std::uniform_real_distribution<double> u(0.0, 1.0), c(-1.0, 1.0);
+    std::vector<fmm::particle3d> p(20000);
+    for (auto& x : p) { x.center = fmm::Vector3d(u(g), u(g), u(g)); }
+    for (auto& x : p) x.q = c(g);
+    fmm::gpu::FastMultipole3d a(p, {10, 32});                // reference conventions (self pair included)
+    fmm::gpu::FastMultipole3d b(p, fmm::Options(10, 32, false));
+    fmm::Options o(7, 48, false);
+    o.adaptive = true;
+    fmm::gpu::FastMultipole3d<double> d(p, o);
+    for (size_t i = 0; i < p.size(); ++i)
+        std::printf("%.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", p[i].center[0], p[i].center[1],
+                    p[i].center[2], p[i].q, a.potentials[i], b.potentials[i], b.forces[i][0], d.potentials[i],
+                    d.forces[i][2]);
+    return 0;
+}
+''')
+    exe = tmp_path / "use"
+    r = subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                        lfmm.LIB_PATH, "-Wl,-rpath," + os.path.dirname(lfmm.LIB_PATH)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    a = np.loadtxt(out.stdout.splitlines())
+    x, q = a[:, :3], a[:, 3]
+    pa, _, ra = lfmm.fmm3d(x, q, terms=11, leaf_max=32)
+    pb, fb, _ = lfmm.fmm3d(x, q, terms=11, leaf_max=32, exclude_self=True)
+    pd, fd, _ = lfmm.fmm3d(x, q, terms=8, leaf_max=48, adaptive=True, exclude_self=True)
+    assert ra["terms"] == 11 and ra["max_leaf"] <= 32
+    assert np.array_equal(pa, a[:, 4]) and np.array_equal(pb, a[:, 5]) and np.array_equal(fb[:, 0], a[:, 6])
+    assert np.array_equal(pd, a[:, 7]) and np.array_equal(fd[:, 2], a[:, 8])
+    # the self pair at the clamped distance: q_i / 1e-4 apart
+    assert np.allclose(a[:, 4] - a[:, 5], q / 1e-4, rtol=1e-9, atol=1e-9)
