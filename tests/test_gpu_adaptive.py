@@ -6,8 +6,8 @@ The reference has uniform-depth trees only; the adaptive solve adds P2L
   the same tree and the same results, bit for bit, as the uniform solve;
 - clustered input against the FP64 direct sum: geometric convergence with
   the number of terms (1.3e-3, 2.4e-5, 8.2e-7, 2.5e-8 at 5, 9, 13, 18 terms);
-- the Plummer sphere of the headline workload (smaller n here) against the
-  direct sum at <= 1e-6, FP64 and mixed.
+- the Plummer sphere of the headline workload, at 300,000 and at the full
+  4,194,304 particles, against the direct sum at <= 1e-6, FP64 and mixed.
 """
 import numpy as np
 import pytest
@@ -81,6 +81,24 @@ def test_adaptive_plummer_vs_direct(lfmm, orc, request, mixed):
            e, ef), flush=True)
     assert r["max_leaf"] <= 64 and r["xw_pairs"] > 0
     assert e <= 1e-6
+
+
+def test_adaptive_plummer_4m_full_size(lfmm, orc, request):
+    """The headline Plummer workload at full size: 4,194,304 particles
+    (a = 0.05, r <= 1), adaptive octree with leaf_max 64, 13 terms, FP64 and
+    mixed, on 4,096 sampled targets against the FP64 direct sum."""
+    case_name(request)
+    n = 4194304
+    x, q = plummer(n, 23)
+    idx = np.random.default_rng(24).choice(n, 4096, replace=False)
+    dp, df = orc.direct3d(x, q, tgt_index=idx, include_self=False)
+    for mixed in (False, True):
+        p, f, r = lfmm.fmm3d(x, q, terms=13, leaf_max=64, adaptive=True, exclude_self=True, mixed=mixed)
+        e, ef = rel_l2(p[idx], dp), rel_l2(f[idx], df)
+        print("plummer 4M mixed=%s depth %d leaves %d boxes %d xw %d: potential %.3e field %.3e" %
+              (mixed, r["tree_depth"], r["num_leaves"], r["num_boxes"], r["xw_pairs"], e, ef), flush=True)
+        assert r["max_leaf"] <= 64 and r["tree_depth"] >= 10
+        assert e <= 1e-6 and ef <= 1e-5
 
 
 @pytest.mark.parametrize("seed,n,leaf_max", [(3, 1500, 8), (8, 6000, 32), (9, 20000, 64)])
