@@ -210,8 +210,9 @@ int lfmm_tree3d(lfmm_ctx* ctx, const lfmm_points3d* particles, int tree_depth, i
  *   every leaf set to the largest request of its particles:
  *   particle_terms[i] (terms N = p + 1 for target i: n_out entries, memory as
  *   the outputs), clamped to [1, 18].  A box with children takes the largest
- *   order of its children; the M2L and L2L into the children of a parent use
- *   the parent's order from level 4 on (levels 2-3 use the largest order).
+ *   order of its children; from level 4 on the M2L and L2L into a box run
+ *   at that box's own order (levels 2-3, P2M and M2M use the largest order,
+ *   the sources of a translation are read up to the target's order).
  *   report->terms is the largest order, report->terms_min the smallest leaf
  *   order.
  *

@@ -386,6 +386,20 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     const double* geom = T.geom;
     const double4* src = T.src;
     const uint32_t* order = T.order;
+    // learned orders: the sampled direct sums start now, beside the solve
+    OrderCheck chk;
+    struct SideDrain {  // no return path leaves the side stream running on this solve's workspace
+        lfmm_ctx* c;
+        const OrderCheck& k;
+        ~SideDrain()
+        {
+            if (k.started) (void)hipStreamSynchronize(c->side_stream);
+        }
+    } drain{ctx, chk};
+    if (learned) {
+        rc = order_check_start(ctx, DP, src, n, n_out, DP.q != nullptr, ctx->order_samples, chk);
+        if (rc != LFMM_OK) return rc;
+    }
     const int32_t* leaf_of = T.leaf_of;
     std::vector<int64_t>& cnt = T.cnt;
     std::vector<uint64_t*>& lkeys = T.lkeys;
@@ -556,7 +570,23 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             L.tiles = (int)((cnt[l - 1] + rtr_tile(N) - 1) / rtr_tile(N));
             const bool bucketed = adaptive && l >= kMixedFirstLevel && (per_box || cnt[l - 1] >= kBucketMinParents);
             if (bucketed) {
-                // one launch per order among the level's parents with children (parent lists)
+                // the level's sources packed once, at the largest order of its buckets
+                int tpack = 0;
+                for (int t = 1; t <= LFMM_MAX_TERMS_3D; ++t)
+                    if (O.seg[((l - 1) * 32 + t) * 8 + 8] > O.seg[((l - 1) * 32 + t) * 8]) tpack = t;
+                if (mixed && tpack > 0) {
+                    M2L16Args mp{};
+                    mp.outer = outer[l];
+                    mp.num_parents = (int)cnt[l - 1];
+                    mp.child_begin = L.child_begin;
+                    mp.child_keys = L.child_keys;
+                    mp.xmax = xmax + l;
+                    mp.xp = xp;
+                    mp.stride = NX;
+                    mp.size = L.size;
+                    LFMM_HIP(launch_m2l16_pack(tpack, mp, st));
+                }
+                // one launch per order among the level's boxes ((parent, class) lists)
                 for (int t = 1; t <= LFMM_MAX_TERMS_3D; ++t) {
                     const int key = ((l - 1) * 32 + t) * 8;
                     const int64_t b0 = O.seg[key], b1 = O.seg[key + 8];
@@ -604,6 +634,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                         for (int c = 0; c < 8; ++c)
                             m.ctile[c + 1] = m.ctile[c] + (m.coff[c + 1] - m.coff[c] + m16_targets(t) - 1) / m16_targets(t);
                         m.tiles = m.ctile[8];
+                        m.xpieces = 4 * ((tpack * tpack + 15) / 16);
                         // a workgroup runs all 189 directions of its targets:
                         // small buckets split the directions into parts so
                         // that about 512 workgroups share the work
@@ -619,7 +650,6 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                             }
                             m.partial = list_partial;
                         }
-                        LFMM_HIP(launch_m2l16_pack(t, m, st));
                         LFMM_HIP(ev_open());
                         LFMM_HIP(launch_rtr(t, 2, DB->rt, Lb, st));
                         LFMM_HIP(ev_close(EV_L2L));
@@ -776,8 +806,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     // learned orders: the sampled direct-sum check judges the run
     double chk_p = 0.0, chk_f = 0.0;
     if (learned) {
-        rc = order_check(ctx, DP, src, n, n_out, DP.q != nullptr, dpot, dfield, !(flags & LFMM_FLAG_EXCLUDE_SELF),
-                         ctx->order_samples, &chk_p, &chk_f);
+        rc = order_check_finish(ctx, chk, dpot, dfield, !(flags & LFMM_FLAG_EXCLUDE_SELF), &chk_p, &chk_f);
         if (rc != LFMM_OK) return rc;
     }
     const bool fail_check = learned && !(chk_p <= ctx->order_tol);
@@ -1090,6 +1119,10 @@ void lfmm_destroy(lfmm_ctx* c)
     if (c->switch_event) (void)hipEventDestroy(c->switch_event);
     for (auto e : c->events) (void)hipEventDestroy(e);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    if (c->side_stream) (void)hipStreamSynchronize(c->side_stream);
+    for (hipEvent_t e : c->side_events)
+        if (e) (void)hipEventDestroy(e);
+    if (c->side_stream) (void)hipStreamDestroy(c->side_stream);
     delete c;
 }
 

@@ -106,6 +106,8 @@ struct lfmm_ctx {
     double order_conf = 0.95;           // cumulative class probability of the chosen order
     double order_tol = 1e-6;            // sampled rel-L2 above which the fixed order reruns the solve
     int order_samples = 256;            // targets of the sampled direct-sum check
+    hipStream_t side_stream = nullptr;  // the sampled check's direct sums, beside the solve
+    hipEvent_t side_events[2] = {nullptr, nullptr};
     std::mutex mu;
 };
 
@@ -205,7 +207,7 @@ int build_adaptive(lfmm_ctx* ctx, int leaf_max, Tree& T);
 struct BoxOrders {
     uint32_t* box_terms = nullptr;   // [total] terms of each box (a parent: the largest of its children)
     int32_t* plist = nullptr;        // per box of the bucketed levels, its parent (index at its level),
-                                     // sorted by (parent level, parent terms, child class)
+                                     // sorted by (parent level, the box's terms, the box's class)
     std::vector<int64_t> seg;        // plist entries of key (level*32 + terms)*8 + class: [seg[key], seg[key+1])
     int max_terms = 0;               // over the leaves
     int min_leaf_terms = 0;
@@ -231,10 +233,25 @@ int order_box_terms(lfmm_ctx* ctx, const Tree& T, const uint32_t* leaf_terms, in
 // Algorithmic FP64 flops of all M2L operations at their orders (levels below
 // first_bucket_level at N).  Synchronises.
 int order_m2l_flops(lfmm_ctx* ctx, const Tree& T, const BoxOrders& O, int N, int first_bucket_level, double* flops);
-// Sampled direct-sum check: rel-L2 of potential (self pair excluded) and field on `samples` targets.
-int order_check(lfmm_ctx* ctx, const DevPoints& DP, const double4* src, int64_t n, int64_t n_out, bool has_q,
-                const double* pot, const double* field, bool fmm_has_self, int samples, double* rel_pot,
-                double* rel_field);
+// Sampled direct-sum check: rel-L2 of potential (self pair excluded) and
+// field on `samples` targets.  order_check_start launches the FP64 direct
+// sums of the sampled targets on the context's side stream once the staged
+// inputs and the packed sources `src` are ready (after the main stream's
+// work so far); order_check_finish waits for them on the main stream,
+// compares with the solve's outputs and synchronises.
+struct OrderCheck {
+    int S = 0;
+    bool started = false;
+    double4* rec = nullptr;
+    int64_t* at = nullptr;
+    double* dp = nullptr;
+    double* df = nullptr;
+    double4* out = nullptr;
+};
+int order_check_start(lfmm_ctx* ctx, const DevPoints& DP, const double4* src, int64_t n, int64_t n_out, bool has_q,
+                      int samples, OrderCheck& C);
+int order_check_finish(lfmm_ctx* ctx, const OrderCheck& C, const double* pot, const double* field, bool fmm_has_self,
+                       double* rel_pot, double* rel_field);
 
 // Near-field pair count of an adaptive tree's U lists and the particles of
 // the lists summed over the leaves (ws: 2 counters).  Synchronises.

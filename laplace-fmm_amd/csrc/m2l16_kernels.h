@@ -483,6 +483,7 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
     // plane entry: class * PIECES * (num_parents + 1) + parent slot (the zero
     // slot num_parents of class 0 when the source or its parent is absent)
     const int64_t np1 = (int64_t)a.num_parents + 1;
+    const int64_t xpc = a.xpieces > 0 ? a.xpieces : PIECES;  // pieces per packed box
     auto box_of = [&](int dir, int ct) -> int64_t {
         int64_t base = a.num_parents;
         if (tgt[ct] >= 0) {
@@ -490,7 +491,7 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
                       sz = (int)tz[ct] + dir_vec[4 * dir + 2];
             if (sx >= 0 && sy >= 0 && sz >= 0 && sx < side && sy < side && sz < side) {
                 const int p = find_box(morton3(sx >> 1, sy >> 1, sz >> 1), a.plut, a.pkeys, a.num_parents);
-                if (p >= 0) base = (int64_t)(((sx & 1) << 2) | ((sy & 1) << 1) | (sz & 1)) * PIECES * np1 + p;
+                if (p >= 0) base = (int64_t)(((sx & 1) << 2) | ((sy & 1) << 1) | (sz & 1)) * xpc * np1 + p;
             }
         }
         return base;

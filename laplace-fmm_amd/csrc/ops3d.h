@@ -154,6 +154,12 @@ struct M2L16Args {
     int csorted = 0;
     int coff[9] = {};
     int ctile[9] = {};
+    // k_m2l16r: 16-byte pieces per packed source box when xp was packed at a
+    // larger order (0: this kernel's own).  The first t^2 real coordinates of
+    // an expansion are those of its order-t truncation, and the operator's
+    // columns beyond them are zero, so one packing serves every order up to
+    // the packed one.
+    int xpieces = 0;
 };
 
 // Matrix-core M2L kernel per expansion size: the row-complete k_m2l16r

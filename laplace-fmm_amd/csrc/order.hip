@@ -15,10 +15,11 @@
 //                 reaches the confidence level
 // The network is trained on the GPU by the kernels below (forward, backward
 // with deterministic per-chunk gradient sums, Adam).  A box with children
-// takes the largest order of its children; the M2L and L2L into the
-// children of a parent run with the parent's order (the kernels take parent
-// lists, one launch per (level, order) bucket), so a leaf's local expansion
-// never has fewer terms than the leaf asked for.
+// takes the largest order of its children; the M2L and L2L into a box run
+// at the box's own order (the kernels take lists of (parent, child class),
+// one launch per (level, order) bucket), so a leaf's local expansion has
+// exactly the terms the leaf asked for and a box's never fewer than any of
+// its descendants.
 #include "api_internal.h"
 #include "device_common.h"
 #include "direct.h"
@@ -358,9 +359,10 @@ __global__ void __launch_bounds__(256) k_box_terms(const int32_t* __restrict__ b
 }
 
 // launch buckets: every box g0 .. g1-1 (levels >= the first bucketed level)
-// under its parent: key ((parent level) * 32 + parent terms) * 8 + child
-// class, value the parent's index at its level.  A (level, terms) bucket is
-// then its parents sorted by the class of the child, one entry per child.
+// under its parent: key ((parent level) * 32 + the box's terms) * 8 + the
+// box's class, value the parent's index at its level.  A (level, terms)
+// bucket is then the boxes of that order as (parent, class) entries sorted
+// by class.
 __global__ void __launch_bounds__(256) k_bucket_keys(const uint32_t* __restrict__ box_terms, int g0, int g1, AdLevels A,
                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                      uint32_t* __restrict__ hist)
@@ -371,14 +373,14 @@ __global__ void __launch_bounds__(256) k_bucket_keys(const uint32_t* __restrict_
     while (l + 1 < A.depth && g >= A.off[l + 1]) ++l;
     const uint64_t k = A.keys[g];
     const int p = od_find(A, l - 1, k >> 3);  // the parent is a kept box
-    const uint32_t key = ((uint32_t)(l - 1) * 32u + box_terms[A.off[l - 1] + p]) * 8u + (uint32_t)(k & 7u);
+    const uint32_t key = ((uint32_t)(l - 1) * 32u + box_terms[g]) * 8u + (uint32_t)(k & 7u);
     keys[g - g0] = key;
     vals[g - g0] = (uint32_t)p;
     atomicAdd(&hist[key], 1u);
 }
 
 // algorithmic M2L flops of one level's targets at their orders: per box,
-// (sources present among its 189 offsets) x flops(terms of its parent)
+// (sources present among its 189 offsets) x flops(its terms)
 struct FlopTable {
     long long f[kMaxTerms3d + 1];
 };
@@ -402,11 +404,7 @@ __global__ void __launch_bounds__(256) k_m2l_flops(AdLevels A, int level, const 
                     if (abs(sx - x) <= 1 && abs(sy - y) <= 1 && abs(sz - z) <= 1) continue;
                     if (od_find(A, level, morton3(sx, sy, sz)) >= 0) ++ops;
                 }
-        int t = fixed_terms;
-        if (box_terms) {
-            const int p = od_find(A, level - 1, k >> 3);
-            t = (int)box_terms[A.off[level - 1] + p];
-        }
+        const int t = box_terms ? (int)box_terms[A.off[level] + i] : fixed_terms;
         c = ops * (unsigned long long)F.f[t];
     }
     for (int off = 32; off > 0; off >>= 1) c += (unsigned long long)__shfl_xor((long long)c, off);
@@ -603,32 +601,53 @@ int order_m2l_flops(lfmm_ctx* ctx, const Tree& T, const BoxOrders& O, int N, int
     return LFMM_OK;
 }
 
-int order_check(lfmm_ctx* ctx, const DevPoints& DP, const double4* src, int64_t n, int64_t n_out, bool has_q,
-                const double* pot, const double* field, bool fmm_has_self, int samples, double* rel_pot,
-                double* rel_field)
+int order_check_start(lfmm_ctx* ctx, const DevPoints& DP, const double4* src, int64_t n, int64_t n_out, bool has_q,
+                      int samples, OrderCheck& C)
 {
     hipStream_t st = ctx->stream;
     Arena& A = ctx->arena;
-    const int S = (int)std::min<int64_t>(samples, n_out);
-    if (S <= 0) {
-        *rel_pot = *rel_field = 0.0;
-        return LFMM_OK;
-    }
-    double4* rec = take<double4>(A, S);
-    int64_t* at = take<int64_t>(A, S);
-    double* dp = take<double>(A, S);
-    double* df = take<double>(A, 3 * S);
-    double4* out = take<double4>(A, S);
+    C = OrderCheck{};
+    C.S = (int)std::min<int64_t>(samples, n_out);
+    if (C.S <= 0) return LFMM_OK;
+    const int S = C.S;
+    C.rec = take<double4>(A, S);
+    C.at = take<int64_t>(A, S);
+    C.dp = take<double>(A, S);
+    C.df = take<double>(A, 3 * S);
+    C.out = take<double4>(A, S);
     const int splits = direct3d_split_count(S, n);
     double4* part = take<double4>(A, (int64_t)splits * S);
-    if (!rec || !at || !dp || !df || !out || !part) return LFMM_ERR_NOMEM;
-    hipLaunchKernelGGL(k_check_gather, dim3(nblk(S, 256)), dim3(256), 0, st, DP.pos, DP.ps, DP.q, DP.qs, n_out, S,
-                       has_q ? 1 : 0, rec, at);
-    LFMM_HIP(launch_direct3d_split(rec, S, src, n, dp, df, part, splits, st));
-    hipLaunchKernelGGL(k_check_diff, dim3(nblk(S, 256)), dim3(256), 0, st, rec, at, S, dp, df, pot, field,
-                       fmm_has_self ? 1 : 0, out);
+    if (!C.rec || !C.at || !C.dp || !C.df || !C.out || !part) return LFMM_ERR_NOMEM;
+    if (!ctx->side_stream) {
+        LFMM_HIP(hipStreamCreateWithFlags(&ctx->side_stream, hipStreamNonBlocking));
+        LFMM_HIP(hipEventCreateWithFlags(&ctx->side_events[0], hipEventDisableTiming));
+        LFMM_HIP(hipEventCreateWithFlags(&ctx->side_events[1], hipEventDisableTiming));
+    }
+    hipStream_t ss = ctx->side_stream;
+    // the sampled direct sums need only the staged inputs: they run on the
+    // side stream while the solve continues on the main one
+    LFMM_HIP(hipEventRecord(ctx->side_events[0], st));
+    LFMM_HIP(hipStreamWaitEvent(ss, ctx->side_events[0], 0));
+    hipLaunchKernelGGL(k_check_gather, dim3(nblk(S, 256)), dim3(256), 0, ss, DP.pos, DP.ps, DP.q, DP.qs, n_out, S,
+                       has_q ? 1 : 0, C.rec, C.at);
+    LFMM_HIP(launch_direct3d_split(C.rec, S, src, n, C.dp, C.df, part, splits, ss));
+    LFMM_HIP(hipEventRecord(ctx->side_events[1], ss));
+    C.started = true;
+    return LFMM_OK;
+}
+
+int order_check_finish(lfmm_ctx* ctx, const OrderCheck& C, const double* pot, const double* field, bool fmm_has_self,
+                       double* rel_pot, double* rel_field)
+{
+    hipStream_t st = ctx->stream;
+    *rel_pot = *rel_field = 0.0;
+    if (!C.started || C.S <= 0) return LFMM_OK;
+    const int S = C.S;
+    LFMM_HIP(hipStreamWaitEvent(st, ctx->side_events[1], 0));
+    hipLaunchKernelGGL(k_check_diff, dim3(nblk(S, 256)), dim3(256), 0, st, C.rec, C.at, S, C.dp, C.df, pot, field,
+                       fmm_has_self ? 1 : 0, C.out);
     std::vector<double4> h(S);
-    LFMM_HIP(hipMemcpyAsync(h.data(), out, sizeof(double4) * S, hipMemcpyDeviceToHost, st));
+    LFMM_HIP(hipMemcpyAsync(h.data(), C.out, sizeof(double4) * S, hipMemcpyDeviceToHost, st));
     LFMM_HIP(hipStreamSynchronize(st));
     double e = 0, r = 0, fe = 0, fr = 0;
     for (const double4& v : h) {

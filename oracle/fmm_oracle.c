@@ -867,6 +867,10 @@ void orc_tree3d_lists(const orc_tree3d* t, int level, int far, int64_t* offsets,
 }
 
 int orc_num_threads(void) { return omp_get_max_threads(); }
+/* Thread count of the oracle's parallel loops; one thread makes every sum's
+ * order fixed (the P2P half lists and the level loops then run in index
+ * order), so results repeat bit for bit. */
+void orc_set_num_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 
 /* ===================================================================== */
 /* 3D solve, multipole3d.cpp:1296-1374                                    */
@@ -887,7 +891,7 @@ static int auto_terms_3d(double eps, int N)
  * reference counterpart): every leaf takes the largest request of its
  * particles (pterms in input order, one per output), a box with children the
  * largest of its children, and the M2L and L2L into a box of level >= 4 run
- * with the order of its parent (levels 2-3 and P2M, M2M, L2P with the
+ * with the box's own order (levels 2-3 and P2M, M2M, L2P with the
  * largest order; coefficients beyond a box's order stay zero). */
 static int fmm3d_impl(int64_t n_src, const double* src_xyz, const double* src_q,
                       int64_t n_tgt, const double* tgt_xyz,
@@ -940,7 +944,7 @@ static int fmm3d_impl(int64_t n_src, const double* src_xyz, const double* src_q,
 					if (bt[l + 1][k] > bt[l][j]) bt[l][j] = bt[l + 1][k];
 	}
 	const int Nx2 = tri(N);
-#define BOX_TERMS(i, j) ((bt && (i) >= 4) ? bt[(i) - 1][tr->lv[i].parent[j]] : N)
+#define BOX_TERMS(i, j) ((bt && (i) >= 4) ? bt[i][j] : N)
 
 	cplx** outer = (cplx**)calloc(depth, sizeof(cplx*));
 	cplx** inner = (cplx**)calloc(depth, sizeof(cplx*));

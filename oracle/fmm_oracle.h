@@ -102,6 +102,7 @@ void orc_direct2d(int64_t n_src, const double* src_xy, const double* src_q,
 void orc_read_error(int64_t n, int dim, const double* exact, const double* res, double* out);
 
 int orc_num_threads(void);
+void orc_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -70,6 +70,7 @@ def lib():
             getattr(L, f).argtypes = [ctypes.c_int, _d, _d, _d]
         L.orc_l2p3d.argtypes = [ctypes.c_int, _d, _d, ctypes.c_int64, _d, _d, _d]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_num_threads.argtypes = [ctypes.c_int]
         L.orc_set_r_eps.argtypes = [ctypes.c_double]
         L.orc_get_r_eps.restype = ctypes.c_double
         L.orc_init()
@@ -332,3 +333,19 @@ class exact_geometry:
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+class threads:
+    """Context: the oracle's parallel loops on n threads (1: every sum in a
+    fixed order, bit-reproducible results)."""
+
+    def __init__(self, n):
+        self.n = n
+
+    def __enter__(self):
+        self._old = lib().orc_num_threads()
+        lib().orc_set_num_threads(int(self.n))
+        return self
+
+    def __exit__(self, *a):
+        lib().orc_set_num_threads(self._old)
