@@ -500,19 +500,26 @@ def fmm3d_vec(xyz, qvec, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max
 
 
 def fmm3d_batch(xyz, q, offsets, eps=1e-8, terms=FMM_AUTO, tree_depth=FMM_AUTO, leaf_max=0, exclude_self=False,
-                mixed=False, workers=0, ctx=None, out=None, reports=False):
+                mixed=False, workers=0, ctx=None, out=None, reports=False, adaptive=False, timings=False,
+                count_ops=False):
     """Many independent single-set 3D systems (lfmm_fmm3d_batch).
 
     xyz (total,3) and q (total,) hold the systems packed one after another;
     system s is rows offsets[s] .. offsets[s+1]-1 (offsets: n_sys + 1 ints).
     Returns (pot (total,), field (total,3)[, list of report dicts]) in the same
     rows.  float64 numpy arrays take the host path, CUDA torch tensors the
-    device path (outputs are then CUDA tensors, or written into out)."""
+    device path (outputs are then CUDA tensors, or written into out).
+    adaptive=True solves every system on its own adaptive octree, all of them
+    as one forest in one solve (tree_depth and workers are then ignored and
+    every report is the forest's); otherwise worker contexts run one
+    uniform-tree solve per system."""
     off = np.ascontiguousarray(offsets, dtype=np.int64)
     n_sys = len(off) - 1
     if n_sys < 0:
         raise ValueError("offsets needs n_sys + 1 entries")
-    flags = (FLAG_EXCLUDE_SELF if exclude_self else 0) | (FLAG_MIXED if mixed else 0)
+    flags = ((FLAG_EXCLUDE_SELF if exclude_self else 0) | (FLAG_MIXED if mixed else 0) |
+             (FLAG_ADAPTIVE if adaptive else 0) | (FLAG_TIMINGS if timings else 0) |
+             (FLAG_COUNT_OPS if count_ops else 0))
     reps = (Report * max(n_sys, 1))() if reports else None
     if _is_torch_cuda(xyz):
         import torch

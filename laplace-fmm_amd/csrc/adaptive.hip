@@ -67,16 +67,16 @@ __global__ void __launch_bounds__(256) k_ad_ubeg(const int32_t* __restrict__ chi
 }
 
 __global__ void __launch_bounds__(256) k_ad_keep(const int32_t* __restrict__ uparent, int64_t cnt,
-                                                 const int32_t* __restrict__ ubeg_parent, int leaf_max,
+                                                 const int32_t* __restrict__ ubeg_parent, int leaf_max, int always,
                                                  uint32_t* __restrict__ keep)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= cnt) return;
     const int p = uparent[i];
-    keep[i] = (ubeg_parent[p + 1] - ubeg_parent[p] > leaf_max) ? 1u : 0u;
+    keep[i] = (always || ubeg_parent[p + 1] - ubeg_parent[p] > leaf_max) ? 1u : 0u;
 }
 
-__global__ void __launch_bounds__(256) k_ad_fill(int last, const uint64_t* __restrict__ ukeys,
+__global__ void __launch_bounds__(256) k_ad_fill(int last, int leafable, const uint64_t* __restrict__ ukeys,
                                                  const int32_t* __restrict__ ubeg, const int32_t* __restrict__ uchild,
                                                  int64_t cnt, const uint32_t* __restrict__ keep,
                                                  const uint32_t* __restrict__ kex, const uint32_t* __restrict__ kex_next,
@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(256) k_ad_fill(int last, const uint64_t* __res
     range_all[g] = make_int2(b, e);
     // a leaf's uniform children are not kept: its child range is empty
     child[a] = last ? 0 : (int32_t)kex_next[uchild[i]];
-    box2leaf[g] = (last || e - b <= leaf_max) ? 0 : -1;
+    box2leaf[g] = (last || (leafable && e - b <= leaf_max)) ? 0 : -1;
 }
 
 __global__ void k_ad_set1(int32_t* __restrict__ p, int32_t v) { *p = v; }
@@ -377,6 +377,21 @@ int bits_for(int64_t v)
     return b;
 }
 
+// A forest leaf's system: its ancestor cell (2a, 2b, 2c) at the system level
+// holds system (a side + b) side + c; out = that system's length scale.
+__global__ void __launch_bounds__(256) k_forest_leaf_scale(const uint64_t* __restrict__ leaf_key,
+                                                           const int32_t* __restrict__ leaf_lvl, int64_t nleaf,
+                                                           int level, int side, const double* __restrict__ sys_scale,
+                                                           double* __restrict__ out)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nleaf) return;
+    const int lv = leaf_lvl[i];
+    const uint64_t k = lv >= level ? leaf_key[i] >> (3 * (lv - level)) : 0;
+    const int a = (int)(compact3(k >> 2) >> 1), b = (int)(compact3(k >> 1) >> 1), c = (int)(compact3(k) >> 1);
+    out[i] = (lv >= level && a < side && b < side && c < side) ? sys_scale[(a * side + b) * side + c] : 1.0;
+}
+
 }  // namespace
 
 }  // namespace lfmm
@@ -411,7 +426,7 @@ static int read_u32(const uint32_t* p, uint32_t* out, hipStream_t st)
     return LFMM_OK;
 }
 
-int build_adaptive(lfmm_ctx* ctx, int leaf_max, Tree& T)
+int build_adaptive(lfmm_ctx* ctx, int leaf_max, Tree& T, int min_leaf_level)
 {
     hipStream_t st = ctx->stream;
     Arena& A = ctx->arena;
@@ -440,7 +455,7 @@ int build_adaptive(lfmm_ctx* ctx, int leaf_max, Tree& T)
             LFMM_HIP(fill_u32(keep[0], 1, 1u, st));
         } else {
             hipLaunchKernelGGL(k_ad_keep, dim3(nblk(T.cnt[l], 256)), dim3(256), 0, st, T.uparent[l], T.cnt[l],
-                               ubeg[l - 1], leaf_max, keep[l]);
+                               ubeg[l - 1], leaf_max, l <= min_leaf_level ? 1 : 0, keep[l]);
         }
         LFMM_HIP(fill_u32(keep[l] + T.cnt[l], 1, 0u, st));
         int rc = scan_total(A, keep[l], kex[l], T.cnt[l], st);
@@ -465,7 +480,8 @@ int build_adaptive(lfmm_ctx* ctx, int leaf_max, Tree& T)
         child[l] = take<int32_t>(A, (int64_t)kept[l] + 1);
         if (!child[l]) return LFMM_ERR_NOMEM;
         const bool lst = l == last;
-        hipLaunchKernelGGL(k_ad_fill, dim3(nblk(T.cnt[l], 256)), dim3(256), 0, st, lst ? 1 : 0, T.lkeys[l], ubeg[l],
+        hipLaunchKernelGGL(k_ad_fill, dim3(nblk(T.cnt[l], 256)), dim3(256), 0, st, lst ? 1 : 0,
+                           l >= min_leaf_level ? 1 : 0, T.lkeys[l], ubeg[l],
                            T.lbegin[l], T.cnt[l], keep[l], kex[l], lst ? nullptr : kex[l + 1], leaf_max, T.loff[l],
                            T.keys_all, T.range_all, child[l], T.box2leaf);
         hipLaunchKernelGGL(k_ad_set1, dim3(1), dim3(1), 0, st, child[l] + kept[l], lst ? 0 : (int32_t)kept[l + 1]);
@@ -632,6 +648,15 @@ int build_adaptive(lfmm_ctx* ctx, int leaf_max, Tree& T)
         hipLaunchKernelGGL(k_ad_near_fill, dim3(nblk(nl, 256)), dim3(256), 0, st, T.leaf_gid, (int)nl, T.u_off, T.u_rng,
                            xo, T.x_rng, g2, T.w_off, wg, T.range_all, kDirectMax, T.n_off, T.n_rng);
     }
+    return LFMM_OK;
+}
+
+int forest_leaf_scale(lfmm_ctx* ctx, const Tree& T, const ForestSpec& fs, double* out)
+{
+    if (T.nleaf > 0)
+        hipLaunchKernelGGL(k_forest_leaf_scale, dim3(nblk(T.nleaf, 256)), dim3(256), 0, ctx->stream, T.leaf_key,
+                           T.leaf_lvl, T.nleaf, fs.level, fs.side, fs.sys_scale, out);
+    LFMM_HIP(hipGetLastError());
     return LFMM_OK;
 }
 

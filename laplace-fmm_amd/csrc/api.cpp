@@ -203,7 +203,7 @@ int stage_inputs(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg,
 // `partition` applies the q = 0 partition of every leaf (morton_tree.cpp:426-468);
 // a single-set solve skips it, since it evaluates every particle either way.
 int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, bool partition, Tree& T,
-               bool keep_parents)
+               bool keep_parents, const double* fixed_geom, int min_depth)
 {
     const int64_t n = DP.n;
     const int dim = DP.dim;
@@ -226,9 +226,11 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
     if (!geom || !part || !nonfinite || !umax) return LFMM_ERR_NOMEM;
     T.geom = geom;
     T.umax = umax;
-    LFMM_HIP(hipMemsetAsync(nonfinite, 0, sizeof(int), st));
-    LFMM_HIP(extent(dim, dpos, ps, n, part, nonfinite, geom, st));
-    {
+    if (fixed_geom) {
+        LFMM_HIP(hipMemcpyAsync(geom, fixed_geom, 4 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else {
+        LFMM_HIP(hipMemsetAsync(nonfinite, 0, sizeof(int), st));
+        LFMM_HIP(extent(dim, dpos, ps, n, part, nonfinite, geom, st));
         int bad = 0;
         LFMM_HIP(hipMemcpyAsync(&bad, nonfinite, sizeof(int), hipMemcpyDeviceToHost, st));
         LFMM_HIP(hipStreamSynchronize(st));
@@ -284,6 +286,7 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
             depth = (int)std::max(3.0, 2 + std::log((double)n / 100.0) / std::log(4.0));
         }
     }
+    depth = std::max(depth, min_depth);
     if (depth < 1 || depth > (dim == 3 ? kMaxDepth : 2 * kMaxDepth - 1)) return LFMM_ERR_DEPTH;
     T.depth = depth;
     const int leaf_level = depth - 1;
@@ -350,7 +353,7 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
 
 int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, double eps, int terms_in,
                 int depth_in, int leaf_max, int memory, unsigned flags, double* pot_out, double* field_out,
-                lfmm_report* rep, const DevTables* tables, const int32_t* req)
+                lfmm_report* rep, const DevTables* tables, const int32_t* req, const ForestSpec* fs)
 {
     const auto T0 = std::chrono::steady_clock::now();
     const bool timed = (flags & LFMM_FLAG_TIMINGS) != 0;
@@ -360,6 +363,11 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     const bool per_box = learned || req;
     if (learned && !ctx->order_model) return LFMM_ERR_INVALID;
     if (per_box && tables) return LFMM_ERR_UNSUPPORTED;
+    if (fs && (per_box || Tg || tables)) return LFMM_ERR_UNSUPPORTED;
+    // first level with local expansions (M2L) and first matrix-core level;
+    // a forest's systems start at fs->level
+    const int first = fs ? fs->level + 2 : 2;
+    const int mfl = fs ? fs->level + kMixedFirstLevel : kMixedFirstLevel;
     int N = terms_3d(eps, terms_in);
     if (!per_box && N < 1) return LFMM_ERR_TERMS;
     hipStream_t st = ctx->stream;
@@ -374,10 +382,11 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     // the adaptive octree splits boxes of more than leaf_max particles (64
     // when not given), down to the depth at which every box holds that many
     if (adaptive && leaf_max <= 0) leaf_max = 64;
-    rc = build_tree(ctx, DP, depth_in, leaf_max, Tg != nullptr, T, adaptive);
+    rc = build_tree(ctx, DP, fs ? LFMM_AUTO : depth_in, leaf_max, Tg != nullptr, T, adaptive, fs ? fs->geom : nullptr,
+                    fs ? fs->level + 2 : 0);
     if (rc != LFMM_OK) return rc;
     if (adaptive) {
-        rc = build_adaptive(ctx, leaf_max, T);
+        rc = build_adaptive(ctx, leaf_max, T, fs ? fs->level : 0);
         if (rc != LFMM_OK) return rc;
         LFMM_HIP(max_run(T.leaf_begin, T.nleaf, T.umax, &T.max_leaf, st));
     }
@@ -416,7 +425,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         uint32_t* leaf_terms = take<uint32_t>(A, T.nleaf);
         if (!leaf_terms) return LFMM_ERR_NOMEM;
         LFMM_HIP(fill_u32(leaf_terms, T.nleaf, (uint32_t)N, st));
-        if ((rc = order_box_terms(ctx, T, leaf_terms, kMixedFirstLevel, O)) != LFMM_OK) return rc;
+        if ((rc = order_box_terms(ctx, T, leaf_terms, mfl, O)) != LFMM_OK) return rc;
     }
     if (per_box) {
         const DevTables* DTf = nullptr;  // direction tables (the same for every N)
@@ -441,11 +450,11 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             rc = order_leaf_terms_from_requests(st, T, dreq, n_out, 1, LFMM_MAX_TERMS_3D, leaf_terms);
             if (rc != LFMM_OK) return rc;
         }
-        if ((rc = order_box_terms(ctx, T, leaf_terms, kMixedFirstLevel, O)) != LFMM_OK) return rc;
+        if ((rc = order_box_terms(ctx, T, leaf_terms, mfl, O)) != LFMM_OK) return rc;
         N = O.max_terms;
         if (N < 1 || N > LFMM_MAX_TERMS_3D) return LFMM_ERR_TERMS;
         // the operator tables of every bucket's order, before any launch
-        for (int l = kMixedFirstLevel; l < depth; ++l)
+        for (int l = mfl; l < depth; ++l)
             for (int t = 1; t <= LFMM_MAX_TERMS_3D; ++t) {
                 const int key = ((l - 1) * 32 + t) * 8;
                 if (O.seg[key + 8] == O.seg[key]) continue;
@@ -464,7 +473,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         if (rc != LFMM_OK) return rc;
     }
     std::vector<double2*> outer(depth, nullptr), inner(depth, nullptr);
-    const bool far = depth >= 3;
+    const bool far = depth >= first + 1;
     size_t nev = 0;
     int m2l_launches = 0, mixed_launches = 0;
     // per-kernel timing (LFMM_FLAG_TIMINGS): event pairs on the solve's
@@ -477,7 +486,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         ev_kind.push_back(kind);
         return hipEventRecord(ctx_event(ctx, nev++), st);
     };
-    const bool mixed = far && (flags & LFMM_FLAG_MIXED) && depth - 1 >= kMixedFirstLevel;
+    const bool mixed = far && (flags & LFMM_FLAG_MIXED) && depth - 1 >= mfl;
     if (mixed && tables) return LFMM_ERR_UNSUPPORTED;  // caller-placed tables hold no matrix-core operators
     uint4* xp = nullptr;
     uint32_t* xmax = nullptr;
@@ -487,7 +496,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         // packed sources of the largest mixed level: box-major (size + 1
         // boxes) or class-major (8 classes x (parents + 1) slots)
         int64_t big = 0;
-        for (int l = kMixedFirstLevel; l < depth; ++l)
+        for (int l = mfl; l < depth; ++l)
             big = std::max(big, m16_rowfull(N) ? 8 * (cnt[l - 1] + 1) : cnt[l] + 1);
         const int64_t nr = (int64_t)N * N, pieces = (nr + 15) / 16 * 4;
         xp = take<uint4>(A, big * pieces);
@@ -517,7 +526,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         P2MArgs pa{src, lbegin[leaf_level], lkeys[leaf_level], (int)cnt[leaf_level], leaf_level, geom, DT->knm,
                    outer[leaf_level]};
         // the matrix-core levels need their largest |coefficient|: the producers max-combine it
-        if (mixed && leaf_level >= kMixedFirstLevel) pa.absmax = xmax + leaf_level;
+        if (mixed && leaf_level >= mfl) pa.absmax = xmax + leaf_level;
         if (adaptive) {  // every leaf of the global list, at its level
             pa.leaf_begin = T.leaf_begin;
             pa.leaf_key = T.leaf_key;
@@ -530,7 +539,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         LFMM_HIP(ev_open());
         LFMM_HIP(launch_p2m(N, pa, st));
         LFMM_HIP(ev_close(EV_P2M));
-        for (int l = leaf_level - 1; l >= 2; --l) {
+        for (int l = leaf_level - 1; l >= first; --l) {
             RtrLevel L{};
             L.level = l;
             L.num_parents = (int)cnt[l];
@@ -538,7 +547,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             L.child_keys = lkeys[l + 1];
             L.src = outer[l + 1];
             L.dst = outer[l];
-            if (mixed && l >= kMixedFirstLevel) L.absmax = xmax + l;
+            if (mixed && l >= mfl) L.absmax = xmax + l;
             LFMM_HIP(ev_open());
             LFMM_HIP(launch_rtr(N, 0, DT->rt, L, st));
             LFMM_HIP(ev_close(EV_M2M));
@@ -555,7 +564,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     double2* list_partial = nullptr;  // direction parts of the matrix-core M2L buckets
     int64_t list_partial_elems = 0;
     if (far) {
-        for (int l = 2; l < depth; ++l) {
+        for (int l = first; l < depth; ++l) {
             RtrLevel L{};
             L.level = l;
             L.num_parents = (int)cnt[l - 1];
@@ -565,10 +574,10 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             L.keys = lkeys[l];
             L.size = (int)cnt[l];
             L.src = outer[l];
-            L.parent_local = (l > 2) ? inner[l - 1] : nullptr;
+            L.parent_local = (l > first) ? inner[l - 1] : nullptr;
             L.dst = inner[l];
             L.tiles = (int)((cnt[l - 1] + rtr_tile(N) - 1) / rtr_tile(N));
-            const bool bucketed = adaptive && l >= kMixedFirstLevel && (per_box || cnt[l - 1] >= kBucketMinParents);
+            const bool bucketed = adaptive && l >= mfl && (per_box || cnt[l - 1] >= kBucketMinParents);
             if (bucketed) {
                 // the level's sources packed once, at the largest order of its buckets
                 int tpack = 0;
@@ -665,7 +674,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                     }
                     ++m2l_launches;
                 }
-            } else if (mixed && l >= kMixedFirstLevel) {
+            } else if (mixed && l >= mfl) {
                 // FP64 L2L into the level, then += the matrix-core M2L
                 M2L16Args m{};
                 m.kmat = DT->m16;
@@ -777,6 +786,13 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         la.outer_all = outer_all;
         la.depth = depth;
         for (int l = 0; l <= depth; ++l) la.loff[l] = T.loff[l];
+        la.local_min = first;
+        if (fs) {
+            double* ls = take<double>(A, T.nleaf);
+            if (!ls) return LFMM_ERR_NOMEM;
+            if ((rc = forest_leaf_scale(ctx, T, *fs, ls)) != LFMM_OK) return rc;
+            la.leaf_scale = ls;
+        }
     } else {
         int2* nbr = take<int2>(A, cnt[leaf_level] * 27);
         if (!nbr) return LFMM_ERR_NOMEM;
@@ -856,9 +872,9 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         rep->m2m_bytes = 0.0;
         rep->l2l_bytes = 0.0;
         if (far) {
-            for (int l = 2; l + 1 < depth; ++l) rep->m2m_bytes += 16.0 * NX * (double)(cnt[l + 1] + cnt[l]);
+            for (int l = first; l + 1 < depth; ++l) rep->m2m_bytes += 16.0 * NX * (double)(cnt[l + 1] + cnt[l]);
             if (mixed)
-                for (int l = kMixedFirstLevel; l < depth; ++l) rep->l2l_bytes += 16.0 * NX * (double)(cnt[l - 1] + cnt[l]);
+                for (int l = mfl; l < depth; ++l) rep->l2l_bytes += 16.0 * NX * (double)(cnt[l - 1] + cnt[l]);
         }
         rep->t_tree_ms = t_tree;
         rep->t_upward_ms = t_up;
@@ -878,10 +894,14 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             }
             rep->m2l_ops_mixed = 0;
             if (mixed)
-                for (int l = kMixedFirstLevel; l < depth; ++l) rep->m2l_ops_mixed += per[l];
+                for (int l = mfl; l < depth; ++l) rep->m2l_ops_mixed += per[l];
+            if (fs) {  // a forest's solve translates within its systems only
+                rep->m2l_ops = 0;
+                for (int l = first; l < depth; ++l) rep->m2l_ops += per[l];
+            }
             rep->m2l_flops = (double)rep->m2l_ops * (double)m2l_flops_per_op(N);
             if (per_box) {
-                rc = order_m2l_flops(ctx, T, O, N, kMixedFirstLevel, &rep->m2l_flops);
+                rc = order_m2l_flops(ctx, T, O, N, mfl, &rep->m2l_flops);
                 if (rc != LFMM_OK) return rc;
             }
         }
@@ -1116,6 +1136,7 @@ void lfmm_destroy(lfmm_ctx* c)
     }
     free_tables2d(c);
     (void)hipFree(c->vec_tmp);
+    (void)hipFree(c->forest_buf);
     if (c->switch_event) (void)hipEventDestroy(c->switch_event);
     for (auto e : c->events) (void)hipEventDestroy(e);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);

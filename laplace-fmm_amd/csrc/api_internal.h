@@ -106,6 +106,8 @@ struct lfmm_ctx {
     double order_conf = 0.95;           // cumulative class probability of the chosen order
     double order_tol = 1e-6;            // sampled rel-L2 above which the fixed order reruns the solve
     int order_samples = 256;            // targets of the sampled direct-sum check
+    void* forest_buf = nullptr;         // lfmm_fmm3d_batch with LFMM_FLAG_ADAPTIVE: the mapped systems
+    size_t forest_bytes = 0;
     hipStream_t side_stream = nullptr;  // the sampled check's direct sums, beside the solve
     hipEvent_t side_events[2] = {nullptr, nullptr};
     std::mutex mu;
@@ -193,8 +195,30 @@ double ms_since(std::chrono::steady_clock::time_point t0);
 int upload_view(lfmm_ctx* ctx, const lfmm_points3d* S, int dim, DevPoints& P);
 // The particle set of a solve: the sources alone, or [targets (q = 0); sources].
 int stage_inputs(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, int memory, int dim, DevPoints& P);
+// fixed_geom: device (shift x, y, z, scale) of the tree instead of the
+// particles' extent (no non-finite check then); min_depth: at least so many
+// levels.
 int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, bool partition, Tree& T,
-               bool keep_parents = false);
+               bool keep_parents = false, const double* fixed_geom = nullptr, int min_depth = 0);
+
+// Many independent systems as one adaptive octree: system s occupies the
+// level-`level` cell (2a, 2b, 2c) of the unit cube, a = s / side^2,
+// b = (s / side) % side, c = s % side, its particles mapped affinely into
+// that cell (forest length = sys_scale[s] x the system's length).  The cells
+// between systems stay empty, so from level + 1 on no U, V, W or X list
+// reaches from one system into another; the solve keeps every box down to
+// `level` (the system cells are never merged into one leaf), runs M2L from
+// level + 2 (a system's level 2) and the matrix-core M2L from level + 4, and
+// the leaf kernels clamp and scale each system's pairs and outputs back to
+// its own units.
+struct ForestSpec {
+    int level = 0;
+    int side = 0;
+    const double* geom = nullptr;       // device (0, 0, 0, 1)
+    const double* sys_scale = nullptr;  // device [side^3]
+};
+// Per leaf of the forest's adaptive octree: sys_scale of its system.
+int forest_leaf_scale(lfmm_ctx* ctx, const Tree& T, const ForestSpec& fs, double* out);
 // W-list boxes of at most this many particles act on a leaf by direct sums
 // instead of M2P (a direct sum costs ~20 flops per source and target, M2P
 // ~20 per coefficient, tri(13) = 91 at 13 terms); X lists into a leaf always
@@ -202,7 +226,9 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
 constexpr int kDirectMax = 96;
 // Turns the uniform tree T (built with keep_parents) into the adaptive octree
 // of leaf size leaf_max and builds its U, X and W lists (adaptive.hip).
-int build_adaptive(lfmm_ctx* ctx, int leaf_max, Tree& T);
+// min_leaf_level > 0: every box above that level is kept and has children
+// (leaves sit on that level or deeper).
+int build_adaptive(lfmm_ctx* ctx, int leaf_max, Tree& T, int min_leaf_level = 0);
 // ---- per-box orders (order.hip)
 struct BoxOrders {
     uint32_t* box_terms = nullptr;   // [total] terms of each box (a parent: the largest of its children)
@@ -262,9 +288,13 @@ int ad_count_pairs(const int32_t* leaf_begin, const uint32_t* u_off, const int2*
 // context's cached ones (built on first use).
 // req: per-box orders requested per particle (input order, memory as the
 // outputs; n_out entries), or null.
+// fs: a forest of independent systems solved as one adaptive octree (see
+// ForestSpec), or null.
+struct ForestSpec;
 int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, double eps, int terms_in,
                 int depth_in, int leaf_max, int memory, unsigned flags, double* pot_out, double* field_out,
-                lfmm_report* rep, const DevTables* tables, const int32_t* req = nullptr);
+                lfmm_report* rep, const DevTables* tables, const int32_t* req = nullptr,
+                const ForestSpec* fs = nullptr);
 // fmm3d_solve's status when a learned-order solve failed its sampled check
 // (the caller reruns it with the fixed order)
 constexpr int kNeedFallback = -2;

@@ -115,6 +115,11 @@ struct LeafArgs {
     int direct_max = 0;                // direct_max particles are in the near lists, not M2P
     int depth = 0;
     int loff[17] = {};
+    int local_min = 2;                 // leaves below this level have no local expansion
+    // forest of independent systems: per leaf, its system's length scale s
+    // (forest length per system length).  The pair clamp is then 1e-4 s, and
+    // the potential and field leave scaled by s and s^2 (the system's units).
+    const double* leaf_scale = nullptr;
 };
 
 // Matrix-core M2L of one level (m2l16_kernels.h).

@@ -682,7 +682,8 @@ __device__ __forceinline__ void l2p_eval(const double2* __restrict__ Lt, const d
 // L2P alone: the far field of every evaluated particle in sorted order.  Run
 // before the pair kernel, which then starts from it, so that the pair loop
 // is not held to the register budget (and occupancy) of the L2P recurrences.
-// ADAPTIVE: leaves sit on several levels (leaf_lvl; no local below level 2),
+// ADAPTIVE: leaves sit on several levels (leaf_lvl; no local below level
+// local_min),
 // and the multipoles of the leaf's W list are evaluated at the particle too
 // (M2P).  M2P by Kelvin inversion: with u the particle relative to the source
 // centre (box units), r = |u| and f, g = l2p_eval of the multipole at
@@ -700,7 +701,7 @@ __global__ void __launch_bounds__(256) k_l2p(LeafArgs a, const double2* __restri
     const double4 t = a.src[s];
     const double scale = a.geom[3];
     double fp = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
-    if (!ADAPTIVE || level >= 2) {
+    if (!ADAPTIVE || level >= a.local_min) {
         const uint32_t ix = compact3(key >> 2), iy = compact3(key >> 1), iz = compact3(key);
         const double cx = box_centre(ix, level, scale, a.geom[0]);
         const double cy = box_centre(iy, level, scale, a.geom[1]);

@@ -111,6 +111,8 @@ __global__ void __launch_bounds__(256) k_p2p_multi(LeafArgs a, const int32_t* __
     int nrc;
     leaf_ranges(a.nbr, a.nbr_off, leaf, nr, nrc);
     const int kl = nrc - 1;  // every list holds the leaf itself: nrc >= 1
+    const double sc = a.leaf_scale ? a.leaf_scale[leaf] : 1.0;
+    const double r2min = 1.e-8 * (sc * sc);  // (1e-4 in the system's units)^2
     // the near ranges walked as one list of total[leaf] sources, the next
     // range's bounds loaded ahead: a wave runs as long as its longest leaf
     // total, not as the sum over the ranges of its longest leaf in each.  The
@@ -131,7 +133,7 @@ __global__ void __launch_bounds__(256) k_p2p_multi(LeafArgs a, const int32_t* __
 #pragma unroll
         for (int u = 0; u < TPL; ++u) {
             const double dx = t[u].x - p.x, dy = t[u].y - p.y, dz = t[u].z - p.z;
-            const double r2 = fmax(fma(dx, dx, fma(dy, dy, dz * dz)), 1.e-8);
+            const double r2 = fmax(fma(dx, dx, fma(dy, dy, dz * dz)), r2min);
             const double ir = rsqrt_nf(r2);
             const double qi = p.w * ir;
             const double qi3 = qi * (ir * ir);
@@ -142,17 +144,18 @@ __global__ void __launch_bounds__(256) k_p2p_multi(LeafArgs a, const int32_t* __
         }
     }
     // the self pair (clamped at 1e-4) was included; its field term is exactly zero
-    const double self = a.exclude_self ? rsqrt_nf(1.e-8) : 0.0;
+    const double self = a.exclude_self ? rsqrt_nf(r2min) : 0.0;
+    const double sc2 = sc * sc;
 #pragma unroll
     for (int u = 0; u < TPL; ++u) {
         if (s0 + u >= e) break;
         const int64_t o = a.perm[s0 + u];
         if (o >= a.n_out) continue;
-        if (a.pot) a.pot[o] = ph[u] - t[u].w * self;
+        if (a.pot) a.pot[o] = (ph[u] - t[u].w * self) * sc;
         if (a.field) {
-            a.field[3 * o] = fx[u];
-            a.field[3 * o + 1] = fy[u];
-            a.field[3 * o + 2] = fz[u];
+            a.field[3 * o] = fx[u] * sc2;
+            a.field[3 * o + 1] = fy[u] * sc2;
+            a.field[3 * o + 2] = fz[u] * sc2;
         }
     }
 }
