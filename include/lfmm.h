@@ -378,6 +378,14 @@ int lfmm_direct3d_vec(lfmm_ctx* ctx, const lfmm_points3d* sources, const lfmm_po
  * device memory the workers first wait for work queued on ctx's stream.  The
  * call returns when every system is done; the first failing status is
  * returned and the remaining systems are not started.
+ * With LFMM_FLAG_ADAPTIVE every system is solved on its own adaptive octree
+ * (leaf_max as lfmm_fmm3d) and all of them in one solve on ctx's stream: the
+ * systems are mapped into separate cells of one cube, each by its own extent
+ * (the reference's rule), with empty cells between them, so that no
+ * interaction list crosses from one system to another; each system's results
+ * equal its own adaptive lfmm_fmm3d solve up to rounding (FP64: rel-L2
+ * ~1e-12).  workers and tree_depth are then ignored, every report holds the
+ * one solve's figures, and LFMM_FLAG_LEARNED_ORDER is not supported.
  */
 #define LFMM_BATCH_DEFAULT_WORKERS 8
 #define LFMM_BATCH_MAX_WORKERS 16
