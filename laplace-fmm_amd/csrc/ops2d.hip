@@ -2,7 +2,7 @@
 //
 //   k_p2m2d    P2M, thread per leaf                    (reference Multipole, src/cpu/multipole.cpp:22-36)
 //   k_m2m2d    M2M, thread per (parent, coefficient)    (M2M :38-53, Upward :155-177)
-//   k_m2l2d    L2L + M2L, thread per (target, coefficient) (M2L :66-84, L2L :86-97, Downward :179-231);
+//   k_m2l2d    L2L + M2L, thread per target, operators by scalar loads (M2L :66-84, L2L :86-97, Downward :179-231);
 //              FP64, or FP32 per-direction sums added in FP64
 //   k_leaf2d   L2P + P2P over the 3x3 neighbourhood, thread per particle,
 //              results scattered to input order       (ComputePotentials / ComputeForces :233-449)
@@ -74,58 +74,89 @@ __global__ void __launch_bounds__(256) k_m2m2d(M2M2dArgs a, const double2* __res
     a.dst[(int64_t)P * N + j] = acc;
 }
 
+// L2L + M2L into the class-c children of 256 consecutive parents, one lane
+// per target (the lane of a parent without a class-c child idles).  All
+// lanes of a workgroup share the class, so the operator rows (the class-c
+// L2L, the M2L of each of the class's 27 directions) sit at wave-uniform
+// addresses and arrive through scalar loads, while each lane keeps its
+// target's N accumulators and one source expansion in registers.  Per
+// (target, coefficient) the products are summed in the order of the
+// reference's Downward (:179-231): L2L, then per direction the level term
+// (coefficient 0) and the source coefficients in increasing order.
 // F32: each direction's sum over the source coefficients is formed in FP32
 // (FP32 operator, sources rounded on load) and added to the FP64 accumulator;
 // the level term and the L2L stay FP64.
-template <bool F32>
+template <int N, bool F32>
 __global__ void __launch_bounds__(256) k_m2l2d(M2L2dArgs a, Tables2d T)
 {
-    const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int N = a.N;
-    if (gid >= (int64_t)a.num_parents * 4 * N) return;
-    const int l = (int)(gid % N);
-    const int64_t rest = gid / N;
-    const int c = (int)(rest & 3), P = (int)(rest >> 2);
+    const int c = blockIdx.x & 3;
+    const int P = (int)(blockIdx.x >> 2) * 256 + (int)threadIdx.x;
     int t = -1;
-    for (int i = a.child_begin[P]; i < a.child_begin[P + 1]; ++i)
-        if ((int)(a.child_keys[i] & 3u) == c) t = i;
-    if (t < 0) return;
+    if (P < a.num_parents)
+        for (int i = a.child_begin[P]; i < a.child_begin[P + 1]; ++i)
+            if ((int)(a.child_keys[i] & 3u) == c) t = i;
+    if (t < 0) return;  // no barriers below: lanes may leave
     const uint64_t key = a.child_keys[t];
     const int tx = (int)compact2(key >> 1), ty = (int)compact2(key);
     const int side = 1 << a.level;
-    double2 acc = make_double2(0.0, 0.0);
+    double2 acc[N];
+#pragma unroll
+    for (int l = 0; l < N; ++l) acc[l] = make_double2(0.0, 0.0);
     if (a.parent_local) {  // L2L from the parent
-        const double2* m = T.l2l + ((int64_t)c * N + l) * N;
-        const double2* x = a.parent_local + (int64_t)P * N;
-        for (int j = l; j < N; ++j) acc = cmadd(acc, m[j], x[j]);
+        const double2* xp = a.parent_local + (int64_t)P * N;
+        double2 x[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) x[j] = xp[j];
+        const double2* m = T.l2l + (int64_t)c * N * N;
+#pragma unroll
+        for (int l = 0; l < N; ++l)
+#pragma unroll
+            for (int j = l; j < N; ++j) acc[l] = cmadd(acc[l], m[l * N + j], x[j]);
     }
     for (int k = 0; k < kDirsPerClass2d; ++k) {
-        const int d = T.class_dirs[c * kDirsPerClass2d + k];
+        const int d = __builtin_amdgcn_readfirstlane(T.class_dirs[c * kDirsPerClass2d + k]);
         const int sx = tx + T.dir_vec[2 * d], sy = ty + T.dir_vec[2 * d + 1];
         if (sx < 0 || sy < 0 || sx >= side || sy >= side) continue;
         const int s = find_box(morton2(sx, sy), a.lut, a.keys, a.size);
         if (s < 0) continue;
-        const double2* x = a.src + (int64_t)s * N;
-        if (l == 0) {  // level term at_0 log h of the normalised M2L
-            acc.x = fma(x[0].x, a.log_h, acc.x);
-            acc.y = fma(x[0].y, a.log_h, acc.y);
-        }
+        const double2* xs = a.src + (int64_t)s * N;
+        double2 x[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) x[j] = xs[j];
+        // level term at_0 log h of the normalised M2L
+        acc[0].x = fma(x[0].x, a.log_h, acc[0].x);
+        acc[0].y = fma(x[0].y, a.log_h, acc[0].y);
         if constexpr (F32) {
-            const float2* m = T.m2l32 + ((int64_t)d * N + l) * N;
-            float ax = 0.f, ay = 0.f;
+            const float2* m = T.m2l32 + (int64_t)d * N * N;
+            float xr[N], xi[N];
+#pragma unroll
             for (int j = 0; j < N; ++j) {
-                const float xr = (float)x[j].x, xi = (float)x[j].y;
-                ax = fmaf(m[j].x, xr, fmaf(-m[j].y, xi, ax));
-                ay = fmaf(m[j].x, xi, fmaf(m[j].y, xr, ay));
+                xr[j] = (float)x[j].x;
+                xi[j] = (float)x[j].y;
             }
-            acc.x += (double)ax;
-            acc.y += (double)ay;
+#pragma unroll
+            for (int l = 0; l < N; ++l) {
+                float ax = 0.f, ay = 0.f;
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    const float2 mv = m[l * N + j];
+                    ax = fmaf(mv.x, xr[j], fmaf(-mv.y, xi[j], ax));
+                    ay = fmaf(mv.x, xi[j], fmaf(mv.y, xr[j], ay));
+                }
+                acc[l].x += (double)ax;
+                acc[l].y += (double)ay;
+            }
         } else {
-            const double2* m = T.m2l + ((int64_t)d * N + l) * N;
-            for (int j = 0; j < N; ++j) acc = cmadd(acc, m[j], x[j]);
+            const double2* m = T.m2l + (int64_t)d * N * N;
+#pragma unroll
+            for (int l = 0; l < N; ++l)
+#pragma unroll
+                for (int j = 0; j < N; ++j) acc[l] = cmadd(acc[l], m[l * N + j], x[j]);
         }
     }
-    a.dst[(int64_t)t * N + l] = acc;
+    double2* out = a.dst + (int64_t)t * N;
+#pragma unroll
+    for (int l = 0; l < N; ++l) out[l] = acc[l];
 }
 
 // Pair term of the reference 2D kernels (src/common/simple_math.h:119-128):
@@ -198,13 +229,25 @@ hipError_t launch_m2m2d(const M2M2dArgs& a, const Tables2d& T, hipStream_t st)
     return hipGetLastError();
 }
 
+template <int N>
+hipError_t m2l2d_n(const M2L2dArgs& a, const Tables2d& T, hipStream_t st)
+{
+    const dim3 grid((unsigned)(4 * ((a.num_parents + 255) / 256)));
+    if (a.f32) hipLaunchKernelGGL((k_m2l2d<N, true>), grid, dim3(256), 0, st, a, T);
+    else hipLaunchKernelGGL((k_m2l2d<N, false>), grid, dim3(256), 0, st, a, T);
+    return hipGetLastError();
+}
+
 hipError_t launch_m2l2d(const M2L2dArgs& a, const Tables2d& T, hipStream_t st)
 {
-    const int64_t n = (int64_t)a.num_parents * 4 * a.N;
-    if (n <= 0) return hipSuccess;
-    if (a.f32) hipLaunchKernelGGL(k_m2l2d<true>, dim3(nblk(n, 256)), dim3(256), 0, st, a, T);
-    else hipLaunchKernelGGL(k_m2l2d<false>, dim3(nblk(n, 256)), dim3(256), 0, st, a, T);
-    return hipGetLastError();
+    if (a.num_parents <= 0) return hipSuccess;
+    switch (a.N) {
+#define X(n) case n: return m2l2d_n<n>(a, T, st);
+        X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15)
+        X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_leaf2d(const Leaf2dArgs& a, hipStream_t st)
