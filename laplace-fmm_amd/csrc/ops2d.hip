@@ -13,6 +13,8 @@
 #include "lfmm_internal.h"
 #include "ops2d.h"
 
+#include <type_traits>
+
 namespace lfmm {
 
 __device__ inline double2 cmadd(double2 acc, double2 a, double2 b)
@@ -75,22 +77,45 @@ __global__ void __launch_bounds__(256) k_m2m2d(M2M2dArgs a, const double2* __res
 }
 
 // L2L + M2L into the class-c children of 256 consecutive parents, one lane
-// per target (the lane of a parent without a class-c child idles).  All
-// lanes of a workgroup share the class, so the operator rows (the class-c
-// L2L, the M2L of each of the class's 27 directions) sit at wave-uniform
-// addresses and arrive through scalar loads, while each lane keeps its
-// target's N accumulators and one source expansion in registers.  Per
-// (target, coefficient) the products are summed in the order of the
-// reference's Downward (:179-231): L2L, then per direction the level term
-// (coefficient 0) and the source coefficients in increasing order.
+// per target, one workgroup per (class, parent tile, block of LB output
+// coefficients).  All lanes of a workgroup share the class and the block, so
+// the operator rows they need (LB rows of the class-c L2L and of the M2L of
+// each of the class's 27 directions) are staged in LDS once and read as
+// broadcasts, while each lane keeps its target's LB accumulators and one
+// source expansion in registers.  The blocks multiply the workgroups by about
+// 4, which the deepest level (one target per lane, 1,024 waves at 65,536
+// targets) needs to hide the source loads.  Per (target, coefficient) the
+// products are summed in the order of the reference's Downward (:179-231):
+// L2L, then per direction the level term (coefficient 0) and the source
+// coefficients in increasing order.
 // F32: each direction's sum over the source coefficients is formed in FP32
 // (FP32 operator, sources rounded on load) and added to the FP64 accumulator;
 // the level term and the L2L stay FP64.
+constexpr int m2l2d_lb(int N) { return (N + 3) / 4; }
+
 template <int N, bool F32>
 __global__ void __launch_bounds__(256) k_m2l2d(M2L2dArgs a, Tables2d T)
 {
+    constexpr int LB = m2l2d_lb(N), NB = (N + LB - 1) / LB, ROWS = LB * N;
+    using op_t = typename std::conditional<F32, float2, double2>::type;
+    __shared__ op_t mop[kDirsPerClass2d * ROWS];
+    __shared__ double2 lop[ROWS];
     const int c = blockIdx.x & 3;
-    const int P = (int)(blockIdx.x >> 2) * 256 + (int)threadIdx.x;
+    const int rest = (int)(blockIdx.x >> 2);
+    const int l0 = (rest % NB) * LB;
+    for (int i = threadIdx.x; i < kDirsPerClass2d * ROWS; i += 256) {
+        const int k = i / ROWS, r = (i % ROWS) / N, j = i % N;
+        const int d = T.class_dirs[c * kDirsPerClass2d + k];
+        const int l = l0 + r;
+        if constexpr (F32) mop[i] = l < N ? T.m2l32[((int64_t)d * N + l) * N + j] : make_float2(0.f, 0.f);
+        else mop[i] = l < N ? T.m2l[((int64_t)d * N + l) * N + j] : make_double2(0.0, 0.0);
+    }
+    for (int i = threadIdx.x; i < ROWS; i += 256) {
+        const int l = l0 + i / N, j = i % N;
+        lop[i] = l < N ? T.l2l[((int64_t)c * N + l) * N + j] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int P = (rest / NB) * 256 + (int)threadIdx.x;
     int t = -1;
     if (P < a.num_parents)
         for (int i = a.child_begin[P]; i < a.child_begin[P + 1]; ++i)
@@ -99,22 +124,22 @@ __global__ void __launch_bounds__(256) k_m2l2d(M2L2dArgs a, Tables2d T)
     const uint64_t key = a.child_keys[t];
     const int tx = (int)compact2(key >> 1), ty = (int)compact2(key);
     const int side = 1 << a.level;
-    double2 acc[N];
+    double2 acc[LB];
 #pragma unroll
-    for (int l = 0; l < N; ++l) acc[l] = make_double2(0.0, 0.0);
+    for (int r = 0; r < LB; ++r) acc[r] = make_double2(0.0, 0.0);
     if (a.parent_local) {  // L2L from the parent
         const double2* xp = a.parent_local + (int64_t)P * N;
         double2 x[N];
 #pragma unroll
         for (int j = 0; j < N; ++j) x[j] = xp[j];
-        const double2* m = T.l2l + (int64_t)c * N * N;
 #pragma unroll
-        for (int l = 0; l < N; ++l)
+        for (int r = 0; r < LB; ++r)
 #pragma unroll
-            for (int j = l; j < N; ++j) acc[l] = cmadd(acc[l], m[l * N + j], x[j]);
+            for (int j = 0; j < N; ++j)
+                if (j >= l0 + r) acc[r] = cmadd(acc[r], lop[r * N + j], x[j]);
     }
     for (int k = 0; k < kDirsPerClass2d; ++k) {
-        const int d = __builtin_amdgcn_readfirstlane(T.class_dirs[c * kDirsPerClass2d + k]);
+        const int d = T.class_dirs[c * kDirsPerClass2d + k];
         const int sx = tx + T.dir_vec[2 * d], sy = ty + T.dir_vec[2 * d + 1];
         if (sx < 0 || sy < 0 || sx >= side || sy >= side) continue;
         const int s = find_box(morton2(sx, sy), a.lut, a.keys, a.size);
@@ -123,11 +148,12 @@ __global__ void __launch_bounds__(256) k_m2l2d(M2L2dArgs a, Tables2d T)
         double2 x[N];
 #pragma unroll
         for (int j = 0; j < N; ++j) x[j] = xs[j];
-        // level term at_0 log h of the normalised M2L
-        acc[0].x = fma(x[0].x, a.log_h, acc[0].x);
-        acc[0].y = fma(x[0].y, a.log_h, acc[0].y);
+        if (l0 == 0) {  // level term at_0 log h of the normalised M2L
+            acc[0].x = fma(x[0].x, a.log_h, acc[0].x);
+            acc[0].y = fma(x[0].y, a.log_h, acc[0].y);
+        }
+        const op_t* m = mop + k * ROWS;
         if constexpr (F32) {
-            const float2* m = T.m2l32 + (int64_t)d * N * N;
             float xr[N], xi[N];
 #pragma unroll
             for (int j = 0; j < N; ++j) {
@@ -135,28 +161,28 @@ __global__ void __launch_bounds__(256) k_m2l2d(M2L2dArgs a, Tables2d T)
                 xi[j] = (float)x[j].y;
             }
 #pragma unroll
-            for (int l = 0; l < N; ++l) {
+            for (int r = 0; r < LB; ++r) {
                 float ax = 0.f, ay = 0.f;
 #pragma unroll
                 for (int j = 0; j < N; ++j) {
-                    const float2 mv = m[l * N + j];
+                    const float2 mv = m[r * N + j];
                     ax = fmaf(mv.x, xr[j], fmaf(-mv.y, xi[j], ax));
                     ay = fmaf(mv.x, xi[j], fmaf(mv.y, xr[j], ay));
                 }
-                acc[l].x += (double)ax;
-                acc[l].y += (double)ay;
+                acc[r].x += (double)ax;
+                acc[r].y += (double)ay;
             }
         } else {
-            const double2* m = T.m2l + (int64_t)d * N * N;
 #pragma unroll
-            for (int l = 0; l < N; ++l)
+            for (int r = 0; r < LB; ++r)
 #pragma unroll
-                for (int j = 0; j < N; ++j) acc[l] = cmadd(acc[l], m[l * N + j], x[j]);
+                for (int j = 0; j < N; ++j) acc[r] = cmadd(acc[r], m[r * N + j], x[j]);
         }
     }
     double2* out = a.dst + (int64_t)t * N;
 #pragma unroll
-    for (int l = 0; l < N; ++l) out[l] = acc[l];
+    for (int r = 0; r < LB; ++r)
+        if (l0 + r < N) out[l0 + r] = acc[r];
 }
 
 // Pair term of the reference 2D kernels (src/common/simple_math.h:119-128):
@@ -232,7 +258,8 @@ hipError_t launch_m2m2d(const M2M2dArgs& a, const Tables2d& T, hipStream_t st)
 template <int N>
 hipError_t m2l2d_n(const M2L2dArgs& a, const Tables2d& T, hipStream_t st)
 {
-    const dim3 grid((unsigned)(4 * ((a.num_parents + 255) / 256)));
+    constexpr int NB = (N + m2l2d_lb(N) - 1) / m2l2d_lb(N);
+    const dim3 grid((unsigned)(4 * NB * ((a.num_parents + 255) / 256)));
     if (a.f32) hipLaunchKernelGGL((k_m2l2d<N, true>), grid, dim3(256), 0, st, a, T);
     else hipLaunchKernelGGL((k_m2l2d<N, false>), grid, dim3(256), 0, st, a, T);
     return hipGetLastError();
