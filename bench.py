@@ -139,10 +139,11 @@ def main():
     ap.add_argument("--no-plummer", action="store_true")
     ap.add_argument("--no-uniform-plummer", action="store_true",
                     help="skip the fixed-depth Plummer solve (2.7 s per solve)")
-    ap.add_argument("--workload", choices=("uniform", "plummer", "2d"), default="uniform",
+    ap.add_argument("--workload", choices=("uniform", "plummer", "2d", "batch"), default="uniform",
                     help="particles of the timed steps (profiling runs): the uniform cube of the metric, the "
-                         "Plummer sphere on the adaptive octree, or (2d) only the 2D solves of 1,048,576 points "
-                         "at p = 20, FP64 then FP32 M2L sums, printed as one JSON line")
+                         "Plummer sphere on the adaptive octree, (2d) only the 2D solves of 1,048,576 points "
+                         "at p = 20, FP64 then FP32 M2L sums, or (batch) only the 512-system batch as one "
+                         "adaptive octree, FP64; the last two print one JSON line of their own")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="one device copy of this many MiB before the steps (calibrates HBM counters)")
     args = ap.parse_args()
@@ -161,6 +162,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "2d":
         print(json.dumps(two_d_steps(args, lfmm, torch, dev)), flush=True)
+        return
+    if args.workload == "batch":
+        print(json.dumps(batch_steps(args, lfmm, torch, dev)), flush=True)
         return
     rng = np.random.default_rng(args.seed)
     x = rng.random((args.n, 3))
@@ -334,9 +338,10 @@ def main():
         pb = torch.empty(len(qb), dtype=torch.float64, device=dev)
         fb = torch.empty((len(qb), 3), dtype=torch.float64, device=dev)
 
-        def run_batch(forest):
+        def run_batch(forest, mix=False):
             def bstep():
-                lfmm.fmm3d_batch(xbd, qbd, offb, terms=9, leaf_max=32, ctx=ctx, out=(pb, fb), adaptive=forest)
+                lfmm.fmm3d_batch(xbd, qbd, offb, terms=9, leaf_max=32, ctx=ctx, out=(pb, fb), adaptive=forest,
+                                 mixed=mix)
             bstep()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -347,6 +352,7 @@ def main():
         # one adaptive octree for all systems, then the worker pool of
         # uniform-tree solves, then a plain loop of single solves
         fms, fsample = run_batch(True)
+        mms, msample = run_batch(True, True)
         wms, wsample = run_batch(False)
         t0 = time.perf_counter()
         for s_ in range(512):
@@ -361,11 +367,14 @@ def main():
                  "particles": int(len(qb)), "ms_per_batch": round(fms, 3), "value": len(qb) / (fms * 1e-3),
                  "unit": "particles/s", "loop_of_single_solves_ms": round(sms, 3),
                  "speedup_vs_loop": round(sms / fms, 2),
+                 "forest_mixed": {"method": "the same forest with the split-FP16 matrix-core M2L at each system's "
+                                           "levels >= 4", "ms_per_batch": round(mms, 3),
+                                 "value": len(qb) / (mms * 1e-3)},
                  "worker_pool": {"method": "8 worker contexts, one uniform-tree solve per system",
                                  "ms_per_batch": round(wms, 3), "value": len(qb) / (wms * 1e-3)},
-                 "samples": {"forest": fsample, "worker_pool": wsample}}
-        log("batch of 512 systems: forest %.3f ms, worker pool %.3f ms, loop of single solves %.3f ms"
-            % (fms, wms, sms))
+                 "samples": {"forest": fsample, "forest_mixed": msample, "worker_pool": wsample}}
+        log("batch of 512 systems: forest %.3f ms (mixed M2L %.3f ms), worker pool %.3f ms, loop of single solves "
+            "%.3f ms" % (fms, mms, wms, sms))
 
     accuracy = None
     other = []
@@ -447,7 +456,7 @@ def main():
                 phys_ = pbh[a_:b_][ix] - qb[a_:b_][ix] * 1e4  # remove the clamped self term q_i / 1e-4
                 eps_.append(float(np.linalg.norm(phys_ - dp_) / np.linalg.norm(dp_)))
                 efs_.append(float(np.linalg.norm(fbh[a_:b_][ix] - df_) / np.linalg.norm(df_)))
-            dst_ = batch if key_ == "forest" else batch["worker_pool"]
+            dst_ = batch if key_ == "forest" else batch[key_]
             dst_["rel_l2_potential_max_over_8_systems"] = max(eps_)
             dst_["rel_l2_field_max_over_8_systems"] = max(efs_)
     for e in extras:
@@ -505,6 +514,29 @@ def two_d_steps(args, lfmm, torch, dev):
         log("2d %s: %.3f ms/step" % ("fp32-m2l" if mix else "fp64", ms))
     ctx.close()
     return out
+
+
+def batch_steps(args, lfmm, torch, dev):
+    """Timed batch solves only (profiling runs): the 512 systems of the
+    batch workload as one adaptive octree (FP64, p = 8, leaf_max 32);
+    args.warmup untimed and args.steps timed batches."""
+    ctx = lfmm.Context(0)
+    xb, qb, offb = batch_systems(512, args.seed + 21)
+    xbd, qbd = torch.from_numpy(xb).to(dev), torch.from_numpy(qb).to(dev)
+    pb = torch.empty(len(qb), dtype=torch.float64, device=dev)
+    fb = torch.empty((len(qb), 3), dtype=torch.float64, device=dev)
+    for _ in range(args.warmup):
+        lfmm.fmm3d_batch(xbd, qbd, offb, terms=9, leaf_max=32, ctx=ctx, out=(pb, fb), adaptive=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lfmm.fmm3d_batch(xbd, qbd, offb, terms=9, leaf_max=32, ctx=ctx, out=(pb, fb), adaptive=True)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / max(args.steps, 1)
+    log("batch forest: %.3f ms per batch" % ms)
+    ctx.close()
+    return {"workload": "512 systems, %d particles, p=8, leaf_max 32, one adaptive octree, FP64" % len(qb),
+            "steps": args.steps, "ms_per_batch": round(ms, 3), "particles_per_s": len(qb) / (ms * 1e-3)}
 
 
 def plummer_leg(args, lfmm, torch, ctx, dev, idx, idx_d, pot, field):

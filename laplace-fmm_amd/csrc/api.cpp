@@ -450,7 +450,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             rc = order_leaf_terms_from_requests(st, T, dreq, n_out, 1, LFMM_MAX_TERMS_3D, leaf_terms);
             if (rc != LFMM_OK) return rc;
         }
-        if ((rc = order_box_terms(ctx, T, leaf_terms, mfl, O)) != LFMM_OK) return rc;
+        if ((rc = order_box_terms(ctx, T, leaf_terms, mfl, O, learned)) != LFMM_OK) return rc;
         N = O.max_terms;
         if (N < 1 || N > LFMM_MAX_TERMS_3D) return LFMM_ERR_TERMS;
         // the operator tables of every bucket's order, before any launch

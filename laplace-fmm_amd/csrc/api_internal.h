@@ -254,8 +254,10 @@ int order_adam(hipStream_t st, float* model, const float* grad, float* m, float*
 // Leaf terms = the largest request of the leaf's particles (requests in input order), clamped to [lo, hi].
 int order_leaf_terms_from_requests(hipStream_t st, const Tree& T, const int32_t* req, int64_t n_req, int lo, int hi,
                                    uint32_t* leaf_terms);
-// Box terms and the (level, terms) launch buckets of the parents of levels >= first_bucket_level.  Synchronises.
-int order_box_terms(lfmm_ctx* ctx, const Tree& T, const uint32_t* leaf_terms, int first_bucket_level, BoxOrders& O);
+// Box terms and the (level, terms) launch buckets of the boxes of levels >= first_bucket_level.  promote:
+// buckets of few boxes take the next larger order of their level (fewer launches).  Synchronises.
+int order_box_terms(lfmm_ctx* ctx, const Tree& T, const uint32_t* leaf_terms, int first_bucket_level, BoxOrders& O,
+                    bool promote = false);
 // Algorithmic FP64 flops of all M2L operations at their orders (levels below
 // first_bucket_level at N).  Synchronises.
 int order_m2l_flops(lfmm_ctx* ctx, const Tree& T, const BoxOrders& O, int N, int first_bucket_level, double* flops);

@@ -379,6 +379,18 @@ __global__ void __launch_bounds__(256) k_bucket_keys(const uint32_t* __restrict_
     atomicAdd(&hist[key], 1u);
 }
 
+// promoted orders: box_terms[g] = prom[(level - 1) * 32 + box_terms[g]] for
+// the boxes of the bucketed levels
+__global__ void __launch_bounds__(256) k_promote_terms(uint32_t* __restrict__ box_terms, int g0, int g1, AdLevels A,
+                                                       const int32_t* __restrict__ prom)
+{
+    const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= g1) return;
+    int l = 0;
+    while (l + 1 < A.depth && g >= A.off[l + 1]) ++l;
+    box_terms[g] = (uint32_t)prom[(l - 1) * 32 + (int)box_terms[g]];
+}
+
 // algorithmic M2L flops of one level's targets at their orders: per box,
 // (sources present among its 189 offsets) x flops(its terms)
 struct FlopTable {
@@ -522,8 +534,10 @@ int order_leaf_terms_from_requests(hipStream_t st, const Tree& T, const int32_t*
     return LFMM_OK;
 }
 
-int order_box_terms(lfmm_ctx* ctx, const Tree& T, const uint32_t* leaf_terms, int first_bucket_level, BoxOrders& O)
+int order_box_terms(lfmm_ctx* ctx, const Tree& T, const uint32_t* leaf_terms, int first_bucket_level, BoxOrders& O,
+                    bool promote)
 {
+    constexpr int64_t kMinBucket = 8192;
     hipStream_t st = ctx->stream;
     Arena& A = ctx->arena;
     const int total = T.total();
@@ -560,13 +574,48 @@ int order_box_terms(lfmm_ctx* ctx, const Tree& T, const uint32_t* leaf_terms, in
         if (!keys || !vals || !ws) return LFMM_ERR_NOMEM;
         hipLaunchKernelGGL(k_bucket_keys, dim3(nblk(m, 256)), dim3(256), 0, st, O.box_terms, g0, g1, T.view(), keys,
                            vals, hist);
+        LFMM_HIP(hipMemcpyAsync(th.data(), hist, sizeof(uint32_t) * nkeys, hipMemcpyDeviceToHost, st));
+        LFMM_HIP(hipStreamSynchronize(st));
+        // A bucket costs two launches whatever its size (about 60 us at 13
+        // terms); one order more costs a target about 20% of its M2L (6 ns).
+        // Buckets of fewer than kMinBucket targets therefore run with the
+        // next larger order present at their level, which only adds terms.
+        std::vector<int32_t> prom((size_t)32 * depth);
+        bool any = false;
+        for (int lp = 0; promote && lp < depth; ++lp) {
+            int64_t count[32] = {};
+            for (int t = 0; t < 32; ++t)
+                for (int c = 0; c < 8; ++c) count[t] += th[(size_t)(lp * 32 + t) * 8 + c];
+            for (int t = 0; t < 32; ++t) prom[(size_t)lp * 32 + t] = t;
+            for (int t = 0; t < 32; ++t) {
+                if (count[t] == 0 || count[t] >= kMinBucket) continue;
+                int u = t + 1;
+                while (u < 32 && count[u] == 0) ++u;
+                if (u >= 32) continue;  // the level's largest order stays
+                count[u] += count[t];
+                count[t] = 0;
+                for (int v = 0; v < 32; ++v)
+                    if (prom[(size_t)lp * 32 + v] == t) prom[(size_t)lp * 32 + v] = u;
+                any = true;
+            }
+        }
+        if (any) {
+            int32_t* dprom = take<int32_t>(A, 32 * depth);
+            if (!dprom) return LFMM_ERR_NOMEM;
+            LFMM_HIP(hipMemcpyAsync(dprom, prom.data(), sizeof(int32_t) * 32 * depth, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_promote_terms, dim3(nblk(m, 256)), dim3(256), 0, st, O.box_terms, g0, g1, T.view(),
+                               dprom);
+            LFMM_HIP(fill_u32(hist, nkeys, 0u, st));
+            hipLaunchKernelGGL(k_bucket_keys, dim3(nblk(m, 256)), dim3(256), 0, st, O.box_terms, g0, g1, T.view(),
+                               keys, vals, hist);
+            LFMM_HIP(hipMemcpyAsync(th.data(), hist, sizeof(uint32_t) * nkeys, hipMemcpyDeviceToHost, st));
+        }
         uint64_t* sk = keys;
         uint32_t* sv = vals;
         LFMM_HIP(radix_sort(keys, vals, m, 16, ws, st, &sk, &sv));
         O.plist = (int32_t*)sv;
     }
     uint32_t hm[2] = {0, 0};
-    LFMM_HIP(hipMemcpyAsync(th.data(), hist, sizeof(uint32_t) * nkeys, hipMemcpyDeviceToHost, st));
     LFMM_HIP(hipMemcpyAsync(hm, mm, sizeof(hm), hipMemcpyDeviceToHost, st));
     LFMM_HIP(hipStreamSynchronize(st));
     O.min_leaf_terms = (int)hm[0];
