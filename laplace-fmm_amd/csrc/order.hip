@@ -537,7 +537,7 @@ int order_leaf_terms_from_requests(hipStream_t st, const Tree& T, const int32_t*
 int order_box_terms(lfmm_ctx* ctx, const Tree& T, const uint32_t* leaf_terms, int first_bucket_level, BoxOrders& O,
                     bool promote)
 {
-    constexpr int64_t kMinBucket = 8192;
+    constexpr int64_t kMinBucket = 1024;
     hipStream_t st = ctx->stream;
     Arena& A = ctx->arena;
     const int total = T.total();
@@ -576,10 +576,10 @@ int order_box_terms(lfmm_ctx* ctx, const Tree& T, const uint32_t* leaf_terms, in
                            vals, hist);
         LFMM_HIP(hipMemcpyAsync(th.data(), hist, sizeof(uint32_t) * nkeys, hipMemcpyDeviceToHost, st));
         LFMM_HIP(hipStreamSynchronize(st));
-        // A bucket costs two launches whatever its size (about 60 us at 13
-        // terms); one order more costs a target about 20% of its M2L (6 ns).
-        // Buckets of fewer than kMinBucket targets therefore run with the
-        // next larger order present at their level, which only adds terms.
+        // A bucket costs two launches whatever its size (tens of us); a
+        // bucket of fewer than kMinBucket targets is dominated by that cost
+        // and runs with the next larger order present at its level instead,
+        // which only adds terms.
         std::vector<int32_t> prom((size_t)32 * depth);
         bool any = false;
         for (int lp = 0; promote && lp < depth; ++lp) {
