@@ -58,15 +58,14 @@ def m2l_flops_per_op(N):
 def m2l16_mfma_flops_per_op(lfmm, N):
     """Matrix-core flops issued per M2L op (one target column, one direction)
     by k_m2l16r: per K chunk of 16 and row tile of 32, one hi*hi MFMA of
-    32x32x16 (2 * 16384 flops for 32 columns) for every chunk, and hi*lo +
-    lo*hi for the chunks below the direction's bound k3 (the largest k3 of
-    its row tiles); averaged over the directions as the 8 child classes use
-    them (each class its 189 offsets)."""
+    32x32x16 (2 * 16384 flops for 32 columns) for the chunks below the row
+    tile's bound k1, and hi*lo + lo*hi for the chunks below its bound k3;
+    averaged over the directions as the 8 child classes use them (each class
+    its 189 offsets)."""
     _, _, kpre = lfmm.host_m2l16_pack(N)  # (316, RT, 2) = (k3, k1)
-    RT = kpre.shape[1]
-    KC = (N * N + 15) // 16
-    k3 = kpre[:, :, 0].max(axis=1)
-    per_dir = (RT * KC + 2 * RT * k3) * 2 * 32 * 16  # flops per column
+    k3 = kpre[:, :, 0].sum(axis=1)
+    k1 = kpre[:, :, 1].sum(axis=1)
+    per_dir = (k1 + 2 * k3) * 2 * 32 * 16  # flops per column
     cd = lfmm.host_op_tables(N, "class_dirs").reshape(8, -1)
     return float(per_dir[cd].mean())
 

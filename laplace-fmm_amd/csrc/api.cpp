@@ -312,6 +312,13 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
     uint32_t* scan_ws = take<uint32_t>(A, scan_ws_elems(n));
     T.leaf_of = take<int32_t>(A, n);
     if (!flags_ws || !excl_ws || !scan_ws || !T.leaf_of) return LFMM_ERR_NOMEM;
+    // Level sizes stay on the device while the levels are compacted: the leaf
+    // level's count is read back once (it bounds the launches above it), then
+    // every coarser level is sized by the bound min(boxes below, 2^(dim l)),
+    // and all counts and the largest leaf come back in one read.
+    uint32_t* cnt_dev = take<uint32_t>(A, depth);
+    if (!cnt_dev) return LFMM_ERR_NOMEM;
+    LFMM_HIP(hipMemsetAsync(cnt_dev, 0, sizeof(uint32_t) * depth, st));
     {
         const int64_t cap = std::min<int64_t>(n, (int64_t)1 << std::min(60, dim * leaf_level));
         T.lkeys[leaf_level] = take<uint64_t>(A, cap);
@@ -320,6 +327,32 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
         LFMM_HIP(run_compact(skeys, n, 0, flags_ws, excl_ws, scan_ws, T.lkeys[leaf_level], T.lbegin[leaf_level],
                              T.leaf_of, &T.cnt[leaf_level], st));
     }
+    if (keep_parents) T.uparent.assign(depth, nullptr);
+    std::vector<int64_t> bound(depth, 0);
+    bound[leaf_level] = T.cnt[leaf_level];
+    for (int l = leaf_level - 1; l >= 0; --l) {
+        const int64_t m = bound[l + 1];
+        bound[l] = std::min<int64_t>(m, (int64_t)1 << (dim * l));
+        T.lkeys[l] = take<uint64_t>(A, bound[l]);
+        T.lbegin[l] = take<int32_t>(A, bound[l] + 1);
+        if (!T.lkeys[l] || !T.lbegin[l]) return LFMM_ERR_NOMEM;
+        if (keep_parents) {
+            T.uparent[l + 1] = take<int32_t>(A, m);
+            if (!T.uparent[l + 1]) return LFMM_ERR_NOMEM;
+        }
+        LFMM_HIP(run_compact_dev(T.lkeys[l + 1], m, l + 1 == leaf_level ? nullptr : cnt_dev + l + 1, dim, flags_ws,
+                                 excl_ws, scan_ws, T.lkeys[l], T.lbegin[l], keep_parents ? T.uparent[l + 1] : nullptr,
+                                 cnt_dev + l, st));
+    }
+    LFMM_HIP(max_run(T.lbegin[leaf_level], T.cnt[leaf_level], umax, nullptr, st));
+    {
+        std::vector<uint32_t> h(depth);
+        LFMM_HIP(hipMemcpyAsync(h.data(), cnt_dev, sizeof(uint32_t) * depth, hipMemcpyDeviceToHost, st));
+        LFMM_HIP(hipMemcpyAsync(&T.max_leaf, umax, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        LFMM_HIP(hipMemcpyAsync(T.geom_host, geom, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+        LFMM_HIP(hipStreamSynchronize(st));
+        for (int l = 0; l < leaf_level; ++l) T.cnt[l] = h[l];
+    }
     if (dq) {
         // q != 0 first inside every leaf, then the sorted sources
         if (partition) LFMM_HIP(partition_leaves(T.lbegin[leaf_level], T.cnt[leaf_level], order, dq, qs, st));
@@ -327,27 +360,12 @@ int build_tree(lfmm_ctx* ctx, const DevPoints& DP, int depth_in, int leaf_max, b
         if (!T.src) return LFMM_ERR_NOMEM;
         LFMM_HIP(gather_sources(dim, dpos, ps, dq, qs, order, n, T.src, st));
     }
-    if (keep_parents) T.uparent.assign(depth, nullptr);
-    for (int l = leaf_level - 1; l >= 0; --l) {
-        const int64_t m = T.cnt[l + 1];
-        const int64_t cap = std::min<int64_t>(m, (int64_t)1 << (dim * l));
-        T.lkeys[l] = take<uint64_t>(A, cap);
-        T.lbegin[l] = take<int32_t>(A, cap + 1);
-        if (!T.lkeys[l] || !T.lbegin[l]) return LFMM_ERR_NOMEM;
-        if (keep_parents) {
-            T.uparent[l + 1] = take<int32_t>(A, m);
-            if (!T.uparent[l + 1]) return LFMM_ERR_NOMEM;
-        }
-        LFMM_HIP(run_compact(T.lkeys[l + 1], m, dim, flags_ws, excl_ws, scan_ws, T.lkeys[l], T.lbegin[l],
-                             keep_parents ? T.uparent[l + 1] : nullptr, &T.cnt[l], st));
-    }
     for (int l = 2; l < depth; ++l) {
         if (l > lut_max_level(dim)) continue;
         T.lut[l] = take<int32_t>(A, (int64_t)1 << (dim * l));
         if (!T.lut[l]) return LFMM_ERR_NOMEM;
         LFMM_HIP(build_lut(dim, T.lkeys[l], T.cnt[l], l, T.lut[l], st));
     }
-    LFMM_HIP(max_run(T.lbegin[leaf_level], T.cnt[leaf_level], umax, &T.max_leaf, st));
     return LFMM_OK;
 }
 

@@ -110,9 +110,7 @@ static int fmm2d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3
     const DevTables2d* DT = nullptr;
     rc = get_tables2d(ctx, N, &DT);
     if (rc != LFMM_OK) return rc;
-    double scale = 0.0;  // box edge at level l = scale / 2^l (for the M2L level term)
-    LFMM_HIP(hipMemcpyAsync(&scale, T.geom + 3, sizeof(double), hipMemcpyDeviceToHost, st));
-    LFMM_HIP(hipStreamSynchronize(st));
+    const double scale = T.geom_host[3];  // box edge at level l = scale / 2^l (for the M2L level term)
 
     std::vector<double2*> outer(depth, nullptr), inner(depth, nullptr);
     const bool far = depth >= 3;

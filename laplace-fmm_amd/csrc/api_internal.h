@@ -137,6 +137,7 @@ struct Tree {
     const double* dq = nullptr;
     int64_t ps = 0, qs = 0;
     double* geom = nullptr;        // {shift x, y, z, scale}
+    double geom_host[4] = {};      // the same, read back with the level sizes
     uint32_t* umax = nullptr;
     uint32_t* order = nullptr;     // sorted -> input index
     double4* src = nullptr;        // sorted (x, y, z, q)

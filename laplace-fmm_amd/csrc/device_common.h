@@ -112,13 +112,20 @@ __device__ inline void wave_absmax(double m, uint32_t* out)
     }
 }
 
-// *out = max(*out, m) over the wave, at most one atomic per wave; skipped
-// when the word already holds at least m (it only grows).  Call from every
-// lane of the wave.
-__device__ inline void wave_max_u32(uint32_t m, uint32_t* out)
+// *out = max(*out, m) over the workgroup (at most 1024 threads), at most one
+// atomic per workgroup; skipped when the word already holds at least m (it
+// only grows).  Call from every thread of the workgroup: a single word
+// takes every atomic, so one per wave serialises thousands of them.
+__device__ inline void block_max_u32(uint32_t m, uint32_t* out)
 {
+    __shared__ uint32_t wm[16];
     for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
-    if ((threadIdx.x & 63) == 0 && m > __atomic_load_n(out, __ATOMIC_RELAXED)) atomicMax(out, m);
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)((blockDim.x + 63) >> 6); ++w) m = max(m, wm[w]);
+        if (m > __atomic_load_n(out, __ATOMIC_RELAXED)) atomicMax(out, m);
+    }
 }
 
 // Find box index by key at a level: dense table when present, otherwise a

@@ -396,11 +396,11 @@ __global__ void __launch_bounds__(256) k_m2l16(M2L16Args a, const uint4* __restr
 // registers) while segment i is multiplied and written to the other LDS
 // buffer afterwards: one barrier per segment, no wave waits for another's
 // rows, and each operator fragment read from LDS feeds 2 * CTW or 4 * CTW
-// MFMAs.  Skipping is per direction instead of per block, which keeps the
-// MFMA stream free of branches: hi*hi for every chunk below the largest k1
-// of the direction's row tiles, hi*lo + lo*hi below the largest k3.  That
-// multiplies a superset of the blocks k_m2l16 multiplies, so the error bound
-// of k_m2l16 holds.
+// MFMAs.  Skipping is per (direction, row tile), as in k_m2l16: hi*hi below
+// the tile's k1, hi*lo + lo*hi below its k3, under scalar branches on the
+// wave-uniform bounds (at 13 terms 96 instead of 131 MFMAs per direction and
+// column tile, averaged over the directions the classes use); the sources
+// are loaded below the largest bounds of the direction.
 template <int N>
 struct M16R {
     static constexpr int RT = M16<N>::RT, KC = M16<N>::KC;
@@ -496,16 +496,19 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
         }
         return base;
     };
-    // bounds of direction dir (wave-uniform): K chunks below k1 need hi*hi,
-    // below k3 all three products, for some row tile
-    auto prefixes = [&](int dir, int& bk3, int& bk1) {
+    // bounds of direction dir (wave-uniform): K chunks below r1[rt] need
+    // hi*hi for row tile rt, below r3[rt] all three products; bk1 and bk3
+    // are their largest values over the row tiles
+    auto prefixes = [&](int dir, int (&r3)[RT], int (&r1)[RT], int& bk3, int& bk1) {
         bk3 = 0;
         bk1 = 0;
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt) {
             const int2 v = kpre[dir * RT + rt];
-            bk3 = max(bk3, __builtin_amdgcn_readfirstlane(v.x));
-            bk1 = max(bk1, __builtin_amdgcn_readfirstlane(v.y));
+            r3[rt] = __builtin_amdgcn_readfirstlane(v.x);
+            r1[rt] = __builtin_amdgcn_readfirstlane(v.y);
+            bk3 = max(bk3, r3[rt]);
+            bk1 = max(bk1, r1[rt]);
         }
     };
     // operator fragments of segment s of direction dir: wave w loads slot
@@ -584,8 +587,8 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
     for (int ct = 0; ct < CTW; ++ct) tot[ct] = m16_acc{};
 
     int dcur = dir_of(d_beg);
-    int bk3, bk1;
-    prefixes(dcur, bk3, bk1);
+    int bk3, bk1, k3r[RT], k1r[RT];
+    prefixes(dcur, k3r, k1r, bk3, bk1);
     int64_t bx[CTW];
 #pragma unroll
     for (int ct = 0; ct < CTW; ++ct) bx[ct] = box_of(dcur, ct);
@@ -604,8 +607,8 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
         int64_t bxn[CTW];
 #pragma unroll
         for (int ct = 0; ct < CTW; ++ct) bxn[ct] = box_of(dnext, ct);
-        int bk3n, bk1n;
-        prefixes(dnext, bk3n, bk1n);
+        int bk3n, bk1n, k3n[RT], k1n[RT];
+        prefixes(dnext, k3n, k1n, bk3n, bk1n);
         // sources of step e + LB (into ring slot (e + LB) % RB)
         auto issue_b = [&](int e) {
             const int e2 = e + LB;
@@ -626,11 +629,11 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
             // segment's MFMAs (written to the other LDS buffer at its end)
             if (s + 1 < NSEG) stg = stage_a(dcur, s + 1, bk3, bk1);
             else stg = stage_a(dnext, 0, bk3n, bk1n);
-            // Per chunk: hi*hi for every row tile (beyond the bound k1 the
-            // sources are the zero box, so those products add nothing), the
-            // two lo products below the bound k3.  The next chunk's hi
-            // fragments and this chunk's lo fragments are read from LDS while
-            // the hi*hi products run; sources arrive two steps ahead.
+            // Per chunk and row tile: hi*hi below the tile's bound k1, the
+            // two lo products below its bound k3 (scalar branches: the
+            // bounds are wave-uniform).  The next chunk's hi fragments and
+            // this chunk's lo fragments are read from LDS while the hi*hi
+            // products run; sources arrive two steps ahead.
             m16_u4 fh[2][RT];
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) fh[0][rt] = cur[(rt * 2) * 64 + lane];
@@ -643,21 +646,25 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
                     const int h = kci & 1;
                     m16_u4 fl[RT];
 #pragma unroll
-                    for (int rt = 0; rt < RT; ++rt) fl[rt] = cur[((kci * RT + rt) * 2 + 1) * 64 + lane];
+                    for (int rt = 0; rt < RT; ++rt)
+                        if (kc < k3r[rt]) fl[rt] = cur[((kci * RT + rt) * 2 + 1) * 64 + lane];
                     if (kci + 1 < SK && kc + NSEG < KC) {
 #pragma unroll
                         for (int rt = 0; rt < RT; ++rt) fh[h ^ 1][rt] = cur[(((kci + 1) * RT + rt) * 2) * 64 + lane];
                     }
 #pragma unroll
                     for (int rt = 0; rt < RT; ++rt)
+                        if (kc < k1r[rt]) {
 #pragma unroll
-                        for (int ct = 0; ct < CTW; ++ct)
-                            acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
-                                __builtin_bit_cast(m16_h8, fh[h][rt]), __builtin_bit_cast(m16_h8, b[ct][0]), acc[rt][ct],
-                                0, 0, 0);
+                            for (int ct = 0; ct < CTW; ++ct)
+                                acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                                    __builtin_bit_cast(m16_h8, fh[h][rt]), __builtin_bit_cast(m16_h8, b[ct][0]),
+                                    acc[rt][ct], 0, 0, 0);
+                        }
                     if (kc < bk3) {
 #pragma unroll
                         for (int rt = 0; rt < RT; ++rt)
+                            if (kc < k3r[rt])
 #pragma unroll
                             for (int ct = 0; ct < CTW; ++ct) {
                                 acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
@@ -682,6 +689,11 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
         dcur = dnext;
         bk3 = bk3n;
         bk1 = bk1n;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+            k3r[rt] = k3n[rt];
+            k1r[rt] = k1n[rt];
+        }
 #pragma unroll
         for (int ct = 0; ct < CTW; ++ct) {
             bx[ct] = bxn[ct];

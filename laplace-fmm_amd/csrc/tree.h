@@ -48,5 +48,11 @@ hipError_t count_ops(int dim, const std::vector<uint64_t*>& keys, const std::vec
 // p[0..n) = v with a fill kernel (safe inside stream capture)
 hipError_t fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t max_run(const int32_t* begin, int64_t count, uint32_t* maxv_dev, uint32_t* out_host, hipStream_t st);
+// run_compact with the counts on the device: at most `bound` elements,
+// *m_dev of them (null: all `bound`); the number of runs goes to *count_dev.
+// No host synchronisation.
+hipError_t run_compact_dev(const uint64_t* keys, int64_t bound, const uint32_t* m_dev, int sh, uint32_t* flags,
+                           uint32_t* excl, uint32_t* scan_ws, uint64_t* ukeys, int32_t* begin, int32_t* owner,
+                           uint32_t* count_dev, hipStream_t st);
 
 }  // namespace lfmm

@@ -126,15 +126,14 @@ template <int D>
 __global__ void __launch_bounds__(256) k_occupancy_up(const uint32_t* __restrict__ fine, int64_t ncoarse,
                                                       uint32_t* __restrict__ coarse, uint32_t* __restrict__ maxv)
 {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     uint32_t m = 0;
-    if (i < ncoarse) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ncoarse; i += (int64_t)gridDim.x * blockDim.x) {
         uint32_t s = 0;
         for (int c = 0; c < (1 << D); ++c) s += fine[i * (1 << D) + c];
         coarse[i] = s;
-        m = s;
+        m = max(m, s);
     }
-    wave_max_u32(m, maxv);
+    block_max_u32(m, maxv);
 }
 
 __global__ void __launch_bounds__(256) k_max_u32(const uint32_t* __restrict__ v, int64_t n, uint32_t* __restrict__ maxv)
@@ -142,7 +141,7 @@ __global__ void __launch_bounds__(256) k_max_u32(const uint32_t* __restrict__ v,
     uint32_t m = 0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         m = max(m, v[i]);
-    wave_max_u32(m, maxv);
+    block_max_u32(m, maxv);
 }
 
 // ------------------------------------------------------------------ scan
@@ -416,6 +415,41 @@ __global__ void __launch_bounds__(256) k_run_compact(const uint64_t* __restrict_
     if (i == m - 1) begin[r + 1] = (int32_t)m;
 }
 
+// The same with the element count on the device (m_dev, at most `bound`
+// elements): the flags and the compaction of a level can follow the previous
+// level's compaction without a host read-back.  *count_dev = runs.
+__global__ void __launch_bounds__(256) k_run_flags_d(const uint64_t* __restrict__ keys, int64_t bound,
+                                                     const uint32_t* __restrict__ m_dev, int sh,
+                                                     uint32_t* __restrict__ flags)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= bound) return;
+    const int64_t m = m_dev ? (int64_t)*m_dev : bound;
+    flags[i] = (i < m && (i == 0 || (keys[i] >> sh) != (keys[i - 1] >> sh))) ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(256) k_run_compact_d(const uint64_t* __restrict__ keys, int64_t bound,
+                                                       const uint32_t* __restrict__ m_dev, int sh,
+                                                       const uint32_t* __restrict__ flags,
+                                                       const uint32_t* __restrict__ excl, uint64_t* __restrict__ ukeys,
+                                                       int32_t* __restrict__ begin, int32_t* __restrict__ owner,
+                                                       uint32_t* __restrict__ count_dev)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t m = m_dev ? (int64_t)*m_dev : bound;
+    if (i >= bound || i >= m) return;
+    const uint32_t r = excl[i] + flags[i] - 1u;
+    if (flags[i]) {
+        ukeys[r] = keys[i] >> sh;
+        begin[r] = (int32_t)i;
+    }
+    if (owner) owner[i] = (int32_t)r;
+    if (i == m - 1) {
+        begin[r + 1] = (int32_t)m;
+        *count_dev = r + 1;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_lut_fill(const uint64_t* __restrict__ ukeys, int64_t count, int32_t* __restrict__ lut)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -432,7 +466,7 @@ __global__ void __launch_bounds__(256) k_max_run(const int32_t* __restrict__ beg
     uint32_t m = 0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
         m = max(m, (uint32_t)(begin[i + 1] - begin[i]));
-    wave_max_u32(m, maxv);
+    block_max_u32(m, maxv);
 }
 
 // ------------------------------------------------------------------ counts
@@ -569,15 +603,15 @@ hipError_t occupancy_depth(int dim, const double* pos, int64_t stride, int64_t n
     else
         hipLaunchKernelGGL(k_occupancy<3>, dim3(nblk(n, 256)), dim3(256), 0, st, pos, stride, n, max_level, geom,
                            counts_ws + offs[max_level]);
-    hipLaunchKernelGGL(k_max_u32, dim3((int)std::min<int64_t>(1024, nblk(nb, 256))), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_max_u32, dim3((int)std::min<int64_t>(256, nblk(nb, 256))), dim3(256), 0, st,
                        counts_ws + offs[max_level], nb, maxv_ws + max_level);
     for (int l = max_level - 1; l >= 0; --l) {
         const int64_t nc = (int64_t)1 << (dim * l);
         if (dim == 2)
-            hipLaunchKernelGGL(k_occupancy_up<2>, dim3(nblk(nc, 256)), dim3(256), 0, st, counts_ws + offs[l + 1], nc,
+            hipLaunchKernelGGL(k_occupancy_up<2>, dim3(std::min(256, nblk(nc, 256))), dim3(256), 0, st, counts_ws + offs[l + 1], nc,
                                counts_ws + offs[l], maxv_ws + l);
         else
-            hipLaunchKernelGGL(k_occupancy_up<3>, dim3(nblk(nc, 256)), dim3(256), 0, st, counts_ws + offs[l + 1], nc,
+            hipLaunchKernelGGL(k_occupancy_up<3>, dim3(std::min(256, nblk(nc, 256))), dim3(256), 0, st, counts_ws + offs[l + 1], nc,
                                counts_ws + offs[l], maxv_ws + l);
     }
     std::vector<uint32_t> mx(max_level + 1);
@@ -676,6 +710,19 @@ hipError_t run_compact(const uint64_t* keys, int64_t m, int sh, uint32_t* flags,
     return hipGetLastError();
 }
 
+hipError_t run_compact_dev(const uint64_t* keys, int64_t bound, const uint32_t* m_dev, int sh, uint32_t* flags,
+                           uint32_t* excl, uint32_t* scan_ws, uint64_t* ukeys, int32_t* begin, int32_t* owner,
+                           uint32_t* count_dev, hipStream_t st)
+{
+    if (bound <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_run_flags_d, dim3(nblk(bound, 256)), dim3(256), 0, st, keys, bound, m_dev, sh, flags);
+    hipError_t e = exclusive_scan(flags, excl, bound, scan_ws, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_run_compact_d, dim3(nblk(bound, 256)), dim3(256), 0, st, keys, bound, m_dev, sh, flags, excl,
+                       ukeys, begin, owner, count_dev);
+    return hipGetLastError();
+}
+
 hipError_t build_lut(int dim, const uint64_t* ukeys, int64_t count, int level, int32_t* lut, hipStream_t st)
 {
     const int64_t span = (int64_t)1 << (dim * level);
@@ -741,8 +788,9 @@ hipError_t max_run(const int32_t* begin, int64_t count, uint32_t* maxv_dev, uint
 {
     hipError_t e = hipMemsetAsync(maxv_dev, 0, 4, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_max_run, dim3((int)std::min<int64_t>(1024, std::max(1, nblk(count, 256)))), dim3(256), 0, st, begin,
+    hipLaunchKernelGGL(k_max_run, dim3((int)std::min<int64_t>(256, std::max(1, nblk(count, 256)))), dim3(256), 0, st, begin,
                        count, maxv_dev);
+    if (!out_host) return hipGetLastError();
     e = hipMemcpyAsync(out_host, maxv_dev, 4, hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(st);
