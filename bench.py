@@ -255,7 +255,8 @@ def main():
                 roofline["traffic"] = t.get("hbm_bytes_per_launch")
                 roofline["traffic_source"] = "profiles/m2l_traffic.json (rocprofv3 FETCH_SIZE, WRITE_SIZE)"
                 if t.get("mfma_busy") is not None:
-                    roofline["mfma_busy_level6"] = t.get("mfma_busy")
+                    roofline["mfma_busy_by_level"] = t.get("mfma_busy")
+                    roofline["mfma_busy_source"] = "profiles/mfma_busy_fmm3d_4m_p12_d7_mixed.md (rocprofv3 SQ_VALU_MFMA_BUSY_CYCLES)"
         except (OSError, ValueError):
             pass
     # HBM rates of the P2P, M2M and L2L kernels on their algorithmic bytes
