@@ -517,17 +517,36 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
     struct Stage {
         m16_u4 v[SFW];
     };
+    // With an even row-tile count, wave w's fragments w + 4 j are (kci, rt,
+    // hl) = (2 j / RT, 2 j % RT + w / 2, w % 2): compile-time in j plus two
+    // per-wave constants, so the per-segment address work is an add and a
+    // bound test per fragment.
+    constexpr bool kEvenRT = (RT % 2) == 0;
+    const int w_hl = w & 1, w_rt = w >> 1;
     auto stage_a = [&](int dir, int s, int k3d, int k1d) -> Stage {
         Stage stg;
+        if constexpr (kEvenRT) {
+            const int kb = w_hl ? k3d : k1d;
+            const int64_t base = ((int64_t)(dir * RT + w_rt) * KC + s) * 2 + w_hl;
 #pragma unroll
-        for (int j = 0; j < SFW; ++j) {
-            const int f = w + 4 * j;
-            if ((SF & 3) == 0 || f < SF) {
-                const int kci = f / (2 * RT), rt = (f >> 1) % RT, hl = f & 1;
+            for (int j = 0; j < SFW; ++j) {
+                const int kci = (2 * j) / RT, rt0 = (2 * j) % RT;
                 const int kc = s + NSEG * kci;
-                const bool need = kc < KC && kc < (hl ? k3d : k1d);
-                const int64_t b = need ? ((int64_t)(dir * RT + rt) * KC + kc) * 2 + hl : 0;
+                const bool need = kc < KC && kc < kb;
+                const int64_t b = need ? base + (int64_t)(rt0 * KC + NSEG * kci) * 2 : 0;
                 stg.v[j] = kv[b * 64 + lane];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < SFW; ++j) {
+                const int f = w + 4 * j;
+                if ((SF & 3) == 0 || f < SF) {
+                    const int kci = f / (2 * RT), rt = (f >> 1) % RT, hl = f & 1;
+                    const int kc = s + NSEG * kci;
+                    const bool need = kc < KC && kc < (hl ? k3d : k1d);
+                    const int64_t b = need ? ((int64_t)(dir * RT + rt) * KC + kc) * 2 + hl : 0;
+                    stg.v[j] = kv[b * 64 + lane];
+                }
             }
         }
         return stg;
