@@ -271,7 +271,8 @@ def main():
            "p2p": dict(rate(p2p_bytes, mean_of(reports, "t_p2p_ms")),
                        kernel="k_p2p_multi<2> with its lane setup",
                        bytes_counted="32 B per near-list source per leaf + 32 B target + 32 B far field + 32 B out"),
-           "m2m": dict(rate(counts["m2m_bytes"], mean_of(reports, "t_m2m_ms")), kernel="k_rtr<N,M2M>",
+           "m2m": dict(rate(counts["m2m_bytes"], mean_of(reports, "t_m2m_ms")),
+                       kernel="k_m2l16r<N,CTW,true> (M2M into levels >= 4 on the matrix cores) + k_rtr<N,M2M> (FP64, levels 2-3)",
                        bytes_counted="every multipole of levels >= 3 read once, levels >= 2 written once"),
            "l2l": dict(rate(counts["l2l_bytes"], mean_of(reports, "t_l2l_ms")), kernel="k_rtr<N,L2L>",
                        bytes_counted="parent locals read once, child locals written once (levels >= 4)")}

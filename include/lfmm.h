@@ -435,6 +435,11 @@ int64_t lfmm_host_op_tables(int N, int which, void* out, int64_t capacity);
  * prefixes [316][ceil(N^2/32)][2] = (k3, k1), or a negative status. */
 int lfmm_host_m2l_dense(int N, int dir, double* out);
 int64_t lfmm_host_m2l16_pack(int N, uint16_t* out, int64_t capacity, double* kinv, int32_t* kpre);
+/* lfmm_host_m2m_dense: the dense real M2M operator from the child of class
+ * `cls` (0..7, bits x y z of its position in the parent) to the parent, on
+ * the normalised coefficients of both (box edges 1/2 and 1), N^2 x N^2
+ * row-major doubles; the matrix-core M2M of LFMM_FLAG_MIXED uses it. */
+int lfmm_host_m2m_dense(int N, int cls, double* out);
 
 /* Library version string. */
 const char* lfmm_version(void);

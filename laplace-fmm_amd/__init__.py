@@ -33,7 +33,7 @@ STATUS = {0: "ok", 1: "invalid argument", 2: "expansion terms out of range", 3: 
 # Every extern "C" entry point declared in include/lfmm.h.
 EXPORTS = ("lfmm_create", "lfmm_destroy", "lfmm_strerror", "lfmm_set_stream", "lfmm_trim", "lfmm_fmm3d",
            "lfmm_tree3d", "lfmm_fmm2d", "lfmm_tree2d", "lfmm_version", "lfmm_host_wigner_d", "lfmm_host_op_tables",
-           "lfmm_host_m2l_dense", "lfmm_host_m2l16_pack", "lfmm_direct3d", "lfmm_direct2d", "lfmm_exact_filename",
+           "lfmm_host_m2l_dense", "lfmm_host_m2m_dense", "lfmm_host_m2l16_pack", "lfmm_direct3d", "lfmm_direct2d", "lfmm_exact_filename",
            "lfmm_write_exact", "lfmm_read_exact", "lfmm_error_stats", "lfmm_fmm3d_vec", "lfmm_direct3d_vec",
            "lfmm_fmm3d_batch", "lfmm_tree3d_adaptive", "lfmm_fmm3d_orders", "lfmm_set_order_model",
            "lfmm_order_features", "lfmm_order_predict", "lfmm_order_loss_grad", "lfmm_order_train",
@@ -126,6 +126,8 @@ def lib():
         L.lfmm_host_op_tables.restype = ctypes.c_int64
         L.lfmm_host_m2l_dense.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         L.lfmm_host_m2l_dense.restype = ctypes.c_int
+        L.lfmm_host_m2m_dense.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        L.lfmm_host_m2m_dense.restype = ctypes.c_int
         L.lfmm_host_m2l16_pack.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                            ctypes.c_void_p]
         L.lfmm_host_m2l16_pack.restype = ctypes.c_int64
@@ -790,6 +792,14 @@ def host_m2l_dense(N, direction):
     """Dense real M2L operator (N^2, N^2) of one direction (index into dir_vec)."""
     out = np.zeros((N * N, N * N))
     _check(lib().lfmm_host_m2l_dense(int(N), int(direction), out.ctypes.data), "lfmm_host_m2l_dense")
+    return out
+
+
+def host_m2m_dense(N, cls):
+    """Dense real M2M operator (N^2, N^2) from the child of class cls (0..7)
+    to its parent, on normalised coefficients."""
+    out = np.zeros((N * N, N * N))
+    _check(lib().lfmm_host_m2m_dense(int(N), int(cls), out.ctypes.data), "lfmm_host_m2m_dense")
     return out
 
 

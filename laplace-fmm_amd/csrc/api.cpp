@@ -85,21 +85,28 @@ int get_m16(lfmm_ctx* ctx, int N, const DevTables** out)
     if (!D.m16) {
         OpTables T;
         build_op_tables(N, T);
-        std::vector<uint16_t> pk;
-        std::vector<int> kp;
-        double kinv = 0.0;
+        std::vector<uint16_t> pk, pm;
+        std::vector<int> kp, km;
+        double kinv = 0.0, kinvm = 0.0;
         pack_m2l16(T, pk, kinv, kp);
-        const size_t bk = (pk.size() * 2 + 255) & ~(size_t)255;
+        pack_m2m16(T, pm, kinvm, km);
+        auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        const size_t bk = al(pk.size() * 2), bkp = al(kp.size() * 4), bm = al(pm.size() * 2);
         char* p = nullptr;
-        if (hipMalloc(&p, bk + kp.size() * 4) != hipSuccess) return LFMM_ERR_NOMEM;
+        if (hipMalloc(&p, bk + bkp + bm + km.size() * 4) != hipSuccess) return LFMM_ERR_NOMEM;
         if (hipMemcpy(p, pk.data(), pk.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(p + bk, kp.data(), kp.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+            hipMemcpy(p + bk, kp.data(), kp.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p + bk + bkp, pm.data(), pm.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p + bk + bkp + bm, km.data(), km.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
             (void)hipFree(p);
             return LFMM_ERR_HIP;
         }
         D.m16 = (uint4*)p;
         D.m16_kpre = (const int2*)(p + bk);
         D.m16_kinv = kinv;
+        D.m2m16 = (const uint4*)(p + bk + bkp);
+        D.m2m16_kpre = (const int2*)(p + bk + bkp + bm);
+        D.m2m16_kinv = kinvm;
     }
     *out = &D;
     return LFMM_OK;
@@ -508,6 +515,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
     if (mixed && tables) return LFMM_ERR_UNSUPPORTED;  // caller-placed tables hold no matrix-core operators
     uint4* xp = nullptr;
     uint32_t* xmax = nullptr;
+    std::vector<uint4*> lpack;  // per-level packed sources (uniform tree), else empty
     if (mixed) {
         rc = get_m16(ctx, N, &DT);
         if (rc != LFMM_OK) return rc;
@@ -521,7 +529,28 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         xmax = take<uint32_t>(A, depth);
         if (!xp || !xmax) return LFMM_ERR_NOMEM;
         LFMM_HIP(fill_u32(xmax, depth, 0u, st));
+        if (!adaptive && m16_rowfull(N)) {
+            // uniform tree: every matrix-core level packed once, when its
+            // multipoles are final, for its M2M into the level above and its
+            // own M2L
+            lpack.assign(depth, nullptr);
+            for (int l = mfl; l < depth; ++l) {
+                lpack[l] = take<uint4>(A, 8 * (cnt[l - 1] + 1) * pieces);
+                if (!lpack[l]) return LFMM_ERR_NOMEM;
+            }
+        }
     }
+    auto pack_level = [&](int l) -> hipError_t {
+        M2L16Args mp{};
+        mp.outer = outer[l];
+        mp.num_parents = (int)cnt[l - 1];
+        mp.child_begin = lbegin[l - 1];
+        mp.child_keys = lkeys[l];
+        mp.xmax = xmax + l;
+        mp.xp = lpack[l];
+        mp.size = (int)cnt[l];
+        return launch_m2l16_pack(N, mp, st);
+    };
     // adaptive octree: the expansions of all boxes in one array each, by gid
     double2* outer_all = nullptr;
     double2* inner_all = nullptr;
@@ -557,7 +586,39 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
         LFMM_HIP(ev_open());
         LFMM_HIP(launch_p2m(N, pa, st));
         LFMM_HIP(ev_close(EV_P2M));
+        if (!lpack.empty() && leaf_level >= mfl) LFMM_HIP(pack_level(leaf_level));
         for (int l = leaf_level - 1; l >= first; --l) {
+            if (!lpack.empty() && l >= mfl) {
+                // M2M into a matrix-core level on the matrix cores: the 8
+                // child classes as the directions, the children's packing
+                M2L16Args mm{};
+                mm.kmat = DT->m2m16;
+                mm.kpre = DT->m2m16_kpre;
+                mm.kinv = DT->m2m16_kinv;
+                mm.xp = lpack[l + 1];
+                mm.xmax = xmax + l + 1;
+                mm.class_dirs = DT->rt.class_dirs;
+                mm.dir_vec = DT->rt.dir_vec;
+                mm.level = l;
+                mm.num_parents = (int)cnt[l - 1];
+                mm.child_begin = lbegin[l - 1];
+                mm.child_keys = lkeys[l];
+                mm.lut = lut[l];
+                mm.keys = lkeys[l];
+                mm.plut = lut[l - 1];
+                mm.pkeys = lkeys[l - 1];
+                mm.size = (int)cnt[l];
+                mm.dst = outer[l];
+                mm.tiles = (int)((cnt[l - 1] + m16_targets(N) - 1) / m16_targets(N));
+                mm.m2m = 1;
+                mm.src_np1 = cnt[l] + 1;
+                mm.absmax = xmax + l;
+                LFMM_HIP(ev_open());
+                LFMM_HIP(launch_m2l16(N, mm, st));
+                LFMM_HIP(ev_close(EV_M2M));
+                LFMM_HIP(pack_level(l));
+                continue;
+            }
             RtrLevel L{};
             L.level = l;
             L.num_parents = (int)cnt[l];
@@ -698,7 +759,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                 m.kmat = DT->m16;
                 m.kpre = DT->m16_kpre;
                 m.outer = outer[l];
-                m.xp = xp;
+                m.xp = lpack.empty() ? xp : lpack[l];
                 m.xmax = xmax + l;
                 m.class_dirs = DT->rt.class_dirs;
                 m.dir_vec = DT->rt.dir_vec;
@@ -724,7 +785,7 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
                     m.partial = take<double2>(A, (int64_t)m.dsplit * cnt[l] * NX);
                     if (!m.partial) return LFMM_ERR_NOMEM;
                 }
-                LFMM_HIP(launch_m2l16_pack(N, m, st));
+                if (lpack.empty()) LFMM_HIP(launch_m2l16_pack(N, m, st));  // else packed in the upward pass
                 LFMM_HIP(ev_open());
                 LFMM_HIP(launch_rtr(N, 2, DT->rt, L, st));
                 LFMM_HIP(ev_close(EV_L2L));
@@ -1308,6 +1369,20 @@ int lfmm_host_m2l_dense(int N, int dir, double* out)
         OpTables T;
         build_op_tables(N, T);
         m2l_dense_matrix(T, dir, out);
+        return LFMM_OK;
+    } catch (...) {
+        return LFMM_ERR_NOMEM;
+    }
+}
+
+int lfmm_host_m2m_dense(int N, int cls, double* out)
+{
+    if (N < 1 || N > LFMM_MAX_TERMS_3D) return LFMM_ERR_TERMS;
+    if (cls < 0 || cls >= 8 || !out) return LFMM_ERR_INVALID;
+    try {
+        OpTables T;
+        build_op_tables(N, T);
+        m2m_dense_matrix(T, cls, out);
         return LFMM_OK;
     } catch (...) {
         return LFMM_ERR_NOMEM;

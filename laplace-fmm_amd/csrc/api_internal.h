@@ -82,6 +82,9 @@ struct DevTables {
     uint4* m16 = nullptr;      // split-fp16 dense M2L operators (built on first mixed solve)
     const int2* m16_kpre = nullptr;  // their block prefixes (same allocation)
     double m16_kinv = 0.0;
+    const uint4* m2m16 = nullptr;      // the same for the 8 M2M operators (same allocation)
+    const int2* m2m16_kpre = nullptr;
+    double m2m16_kinv = 0.0;
 };
 
 

@@ -85,6 +85,10 @@ void m2l_dense_matrix(const OpTables& T, int dir, double* K);
 // (direction, row tile of 32) the block prefixes (k3, k1): K chunks below k3
 // need all three split products, below k1 the hi*hi product alone.
 void pack_m2l16(const OpTables& T, std::vector<uint16_t>& out, double& kinv, std::vector<int>& kpre);
+// The same for the M2M from the child of class c (c = 0..7, the 8
+// "directions"; reference M2M, multipole3d.cpp:115-150), under its own scale.
+void m2m_dense_matrix(const OpTables& T, int cls, double* K);
+void pack_m2m16(const OpTables& T, std::vector<uint16_t>& out, double& kinv, std::vector<int>& kpre);
 
 // Small host helpers exposed for tests through the C ABI debug entry points.
 double host_knm(int n, int m);

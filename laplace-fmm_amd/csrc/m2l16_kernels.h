@@ -414,7 +414,7 @@ struct M16R {
 // per lane) leave room for the operand pipeline, else one.
 constexpr int m16r_ctw(int N) { return m16r_max_rt2 >= (N * N + 31) / 32 ? 2 : 1; }
 
-template <int N, int CTW>
+template <int N, int CTW, bool M2M = false>
 __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, const uint4* __restrict__ kmat, const uint4* __restrict__ xp,
                                                 const int* __restrict__ class_dirs, const int* __restrict__ dir_vec,
                                                 const int2* __restrict__ kpre)
@@ -451,7 +451,9 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
         part = (blk - tile * 8 * a.dsplit) >> 3;
         c = blk & 7;
     }
-    const int d_beg = part * kDirsPerClass / a.dsplit, d_end = (part + 1) * kDirsPerClass / a.dsplit;
+    // M2L: the class's 189 offsets; M2M: the 8 child classes
+    const int ndir = M2M ? 8 : kDirsPerClass;
+    const int d_beg = part * ndir / a.dsplit, d_end = (part + 1) * ndir / a.dsplit;
     const int side = 1 << a.level;
 
     // targets of this lane: the class-c child of parent tile * TC + column,
@@ -478,14 +480,20 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
             tz[ct] = compact3(k);
         }
     }
-    auto dir_of = [&](int d) { return class_dirs[c * kDirsPerClass + d]; };
+    auto dir_of = [&](int d) { return M2M ? d : class_dirs[c * kDirsPerClass + d]; };
     // source of column tile ct in direction dir, as the offset of its class
     // plane entry: class * PIECES * (num_parents + 1) + parent slot (the zero
     // slot num_parents of class 0 when the source or its parent is absent)
-    const int64_t np1 = (int64_t)a.num_parents + 1;
+    const int64_t np1 = M2M ? a.src_np1 : (int64_t)a.num_parents + 1;
+    const int64_t zslot = np1 - 1;  // the zero entry
     const int64_t xpc = a.xpieces > 0 ? a.xpieces : PIECES;  // pieces per packed box
     auto box_of = [&](int dir, int ct) -> int64_t {
-        int64_t base = a.num_parents;
+        int64_t base = zslot;
+        if constexpr (M2M) {
+            // the target's child of class dir, packed by its parent (the target)
+            if (tgt[ct] >= 0) base = (int64_t)dir * xpc * np1 + tgt[ct];
+            return base;
+        }
         if (tgt[ct] >= 0) {
             const int sx = (int)tx[ct] + dir_vec[4 * dir], sy = (int)ty[ct] + dir_vec[4 * dir + 1],
                       sz = (int)tz[ct] + dir_vec[4 * dir + 2];
@@ -565,7 +573,7 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
         const int64_t ph = (int64_t)(kc * 4 + (lane >> 5)) * np1;
 #pragma unroll
         for (int ct = 0; ct < CTW; ++ct) {
-            const int64_t bh = kc < bk1 ? box[ct] : (int64_t)a.num_parents;
+            const int64_t bh = kc < bk1 ? box[ct] : zslot;
             b[ct][0] = xv[bh + ph];
         }
         if (kc < bk3) {
@@ -732,11 +740,14 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
     const int64_t part_size = a.csorted ? a.plist_count : a.size;
     double2* const dst = a.dsplit > 1 ? a.partial + (int64_t)part * part_size * tri(N) : a.dst;
     const int64_t SX = (a.dsplit == 1 && a.stride > 0) ? a.stride : tri(N);
+    // written whole: a direction part's partial sum, or an M2M result
+    const bool assign = a.dsplit > 1 || M2M;
+    double mx = 0.0;  // M2M: largest |coefficient| written
 #pragma unroll
     for (int ct = 0; ct < CTW; ++ct) {
         if (tgt[ct] < 0) continue;
         double* out = (double*)(dst + (int64_t)(a.dsplit > 1 && a.csorted ? pidx[ct] : tgt[ct]) * SX);
-        if (a.dsplit > 1) {
+        if (assign) {
             // lanes l and l + 32 hold the same target
             for (int n = lane >> 5; n < N; n += 2) out[2 * tri(n) + 1] = 0.0;
         }
@@ -748,12 +759,15 @@ __global__ void __launch_bounds__(256, CTW == 1 ? 2 : 1) k_m2l16r(M2L16Args a, c
                 if (row < NR) {
                     const int o = m16_coef_offset(row);
                     const double val = (double)(rt == 0 ? tot[ct][v] : acc[rt][ct][v]) * inv;
-                    if (a.dsplit > 1) out[o] = val;
+                    if (assign) out[o] = val;
                     else out[o] += val;
+                    mx = fmax(mx, fabs(val));
                 }
             }
         }
     }
+    if constexpr (M2M)
+        if (a.absmax) wave_absmax(mx, a.absmax);
 }
 
 // Targets per workgroup = 32 * m16_ct(N): two column tiles while the
@@ -798,11 +812,17 @@ hipError_t do_m2l16(const M2L16Args& a, hipStream_t st)
         return hipGetLastError();
     }
     if (tiles != a.tiles || a.dsplit < 1 || a.dsplit > kDirsPerClass) return hipErrorInvalidValue;
+    if (a.m2m && (!m16_rowfull(N) || a.dsplit != 1 || a.plist || a.stride || a.src_np1 < 1)) return hipErrorInvalidValue;
     if (a.plist && (a.dsplit != 1 || !m16_rowfull(N))) return hipErrorInvalidValue;
     if (tiles > 0) {
-        if constexpr (m16_rowfull(N))
-            hipLaunchKernelGGL((k_m2l16r<N, m16r_ctw(N)>), dim3(tiles * 8 * a.dsplit), dim3(256), 0, st, a, a.kmat,
-                               (const uint4*)a.xp, a.class_dirs, a.dir_vec, a.kpre);
+        if constexpr (m16_rowfull(N)) {
+            if (a.m2m)
+                hipLaunchKernelGGL((k_m2l16r<N, m16r_ctw(N), true>), dim3(tiles * 8), dim3(256), 0, st, a, a.kmat,
+                                   (const uint4*)a.xp, a.class_dirs, a.dir_vec, a.kpre);
+            else
+                hipLaunchKernelGGL((k_m2l16r<N, m16r_ctw(N)>), dim3(tiles * 8 * a.dsplit), dim3(256), 0, st, a, a.kmat,
+                                   (const uint4*)a.xp, a.class_dirs, a.dir_vec, a.kpre);
+        }
         else
             hipLaunchKernelGGL((k_m2l16<N, m16_ct(N)>), dim3(tiles * 8 * a.dsplit), dim3(256), 0, st, a, a.kmat,
                                (const uint4*)a.xp, a.class_dirs, a.dir_vec, a.kpre);

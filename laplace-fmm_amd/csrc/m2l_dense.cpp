@@ -24,15 +24,18 @@ namespace lfmm {
 using ld = long double;
 using cld = std::complex<ld>;
 
-// Applies the normalised FP64 operator of M2L direction `dir` (tables of T)
-// to one expansion, in long double, in the order of the FP64 kernel's phases.
-static void apply_m2l(const OpTables& T, int dir, const cld* in, cld* out)
+// Applies the normalised FP64 rotate-translate-rotate operator of direction
+// `dir` with translation table `tsel` (tables of T) to one expansion, in long
+// double, in the order of the FP64 kernel's phases.  M2L: dir < kNumM2LDirs
+// and its own distance class; M2M from the child of class c: dir =
+// kDiagBase + c and the M2M table (reference M2M, multipole3d.cpp:115-150).
+static void apply_rtr(const OpTables& T, int dir, int tsel, const cld* in, cld* out)
 {
     const int N = T.N;
     const int RS = rot_size(N), TS = tr_size(N);
     const double* rot = &T.rot[(size_t)T.dir_theta[dir] * RS * 2];
     const double* eph = &T.ephi[(size_t)dir * N * 2];
-    const double* tt = &T.trans[(size_t)T.dir_trans[dir] * TS];
+    const double* tt = &T.trans[(size_t)tsel * TS];
     std::vector<cld> s(tri(N));
     // Rz(phi) then Ry(theta), per degree: (P, Q) act on (Re, Im)
     for (int n = 0; n < N; ++n) {
@@ -76,7 +79,7 @@ static void apply_m2l(const OpTables& T, int dir, const cld* in, cld* out)
 
 int m2l_real_index(int n, int m, int part) { return n * n + (m == 0 ? 0 : 2 * m - 1 + part); }
 
-void m2l_dense_matrix(const OpTables& T, int dir, double* K)
+static void rtr_dense_matrix(const OpTables& T, int dir, int tsel, double* K)
 {
     const int N = T.N, NR = N * N;
     std::vector<cld> in(tri(N)), out(tri(N));
@@ -86,7 +89,7 @@ void m2l_dense_matrix(const OpTables& T, int dir, double* K)
         const int r = col - n * n;
         const int m = (r + 1) / 2;
         in[tri(n) + m] = (r == 0 || (r & 1)) ? cld(1, 0) : cld(0, 1);
-        apply_m2l(T, dir, in.data(), out.data());
+        apply_rtr(T, dir, tsel, in.data(), out.data());
         for (int j = 0; j < N; ++j)
             for (int mu = 0; mu <= j; ++mu) {
                 K[(size_t)m2l_real_index(j, mu, 0) * NR + col] = (double)out[tri(j) + mu].real();
@@ -94,6 +97,10 @@ void m2l_dense_matrix(const OpTables& T, int dir, double* K)
             }
     }
 }
+
+void m2l_dense_matrix(const OpTables& T, int dir, double* K) { rtr_dense_matrix(T, dir, T.dir_trans[dir], K); }
+
+void m2m_dense_matrix(const OpTables& T, int cls, double* K) { rtr_dense_matrix(T, kDiagBase + cls, kTransM2M, K); }
 
 static uint16_t half_bits(float x)
 {
@@ -103,22 +110,21 @@ static uint16_t half_bits(float x)
     return b;
 }
 
-void pack_m2l16(const OpTables& T, std::vector<uint16_t>& out, double& kinv, std::vector<int>& kpre)
+// Splits nd dense operators K (nd x NR x NR) into the fragment layout and
+// block prefixes of the matrix-core kernel, under one power-of-two scale.
+static void pack16(int N, int nd, const std::vector<double>& K, std::vector<uint16_t>& out, double& kinv,
+                   std::vector<int>& kpre)
 {
-    const int N = T.N, NR = N * N;
+    const int NR = N * N;
     const int RT = (NR + 31) / 32, KC = (NR + 15) / 16;
-    std::vector<double> K((size_t)kNumM2LDirs * NR * NR);
     double kmax = 0.0;
-    for (int d = 0; d < kNumM2LDirs; ++d) {
-        m2l_dense_matrix(T, d, &K[(size_t)d * NR * NR]);
-    }
     for (double v : K) kmax = std::max(kmax, std::fabs(v));
     int e = 0;
     std::frexp(kmax, &e);  // kmax < 2^e
     const double sk = std::ldexp(1.0, 14 - e);
     kinv = 1.0 / sk;
-    out.assign((size_t)kNumM2LDirs * RT * KC * 2 * 64 * 8, 0);
-    for (int d = 0; d < kNumM2LDirs; ++d)
+    out.assign((size_t)nd * RT * KC * 2 * 64 * 8, 0);
+    for (int d = 0; d < nd; ++d)
         for (int rt = 0; rt < RT; ++rt)
             for (int kc = 0; kc < KC; ++kc)
                 for (int lane = 0; lane < 64; ++lane)
@@ -135,9 +141,9 @@ void pack_m2l16(const OpTables& T, std::vector<uint16_t>& out, double& kinv, std
     // block prefixes: a 32 x 16 block whose largest |entry| is below 2^-11 of
     // kmax keeps hi*hi only (its lo part and the hi*lo product are below
     // 2^-22 kmax max|X|); below 2^-22 kmax it contributes nothing measurable
-    kpre.assign((size_t)kNumM2LDirs * RT * 2, 0);
+    kpre.assign((size_t)nd * RT * 2, 0);
     const double t3 = std::ldexp(kmax, -11), t1 = std::ldexp(kmax, -22);
-    for (int d = 0; d < kNumM2LDirs; ++d)
+    for (int d = 0; d < nd; ++d)
         for (int rt = 0; rt < RT; ++rt) {
             int k3 = 0, k1 = 0;
             for (int kc = 0; kc < KC; ++kc) {
@@ -151,6 +157,22 @@ void pack_m2l16(const OpTables& T, std::vector<uint16_t>& out, double& kinv, std
             kpre[((size_t)d * RT + rt) * 2] = k3;
             kpre[((size_t)d * RT + rt) * 2 + 1] = std::max(k1, k3);
         }
+}
+
+void pack_m2l16(const OpTables& T, std::vector<uint16_t>& out, double& kinv, std::vector<int>& kpre)
+{
+    const int NR = T.N * T.N;
+    std::vector<double> K((size_t)kNumM2LDirs * NR * NR);
+    for (int d = 0; d < kNumM2LDirs; ++d) m2l_dense_matrix(T, d, &K[(size_t)d * NR * NR]);
+    pack16(T.N, kNumM2LDirs, K, out, kinv, kpre);
+}
+
+void pack_m2m16(const OpTables& T, std::vector<uint16_t>& out, double& kinv, std::vector<int>& kpre)
+{
+    const int NR = T.N * T.N;
+    std::vector<double> K((size_t)8 * NR * NR);
+    for (int c = 0; c < 8; ++c) m2m_dense_matrix(T, c, &K[(size_t)c * NR * NR]);
+    pack16(T.N, 8, K, out, kinv, kpre);
 }
 
 }  // namespace lfmm

@@ -165,6 +165,17 @@ struct M2L16Args {
     // columns beyond them are zero, so one packing serves every order up to
     // the packed one.
     int xpieces = 0;
+    // M2M mode (k_m2l16r only): the targets are the boxes of `level` (the
+    // class-c children of the num_parents boxes above them, as for M2L), the
+    // 8 "directions" are the M2M operators from their children of class
+    // 0..7 (kmat / kpre / kinv of the M2M pack), and xp is the class-major
+    // packing of the children's level by target: entry (class, piece, t) at
+    // xp[(class * pieces + piece) * src_np1 + t], slot src_np1 - 1 zero.
+    // The result is written (not added) to dst, and its largest |coefficient|
+    // goes to *absmax (float bits).
+    int m2m = 0;
+    int64_t src_np1 = 0;
+    uint32_t* absmax = nullptr;
 };
 
 // Matrix-core M2L kernel per expansion size: the row-complete k_m2l16r

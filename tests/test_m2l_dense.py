@@ -122,3 +122,25 @@ def test_m2l16_block_prefixes_bound_dropped_terms(lfmm, N):
                 assert B[rt, k1 - 1] >= kmax * 2.0 ** -22 or k1 == k3
             if k3 > 0:
                 assert B[rt, k3 - 1] >= kmax * 2.0 ** -11
+
+
+@pytest.mark.parametrize("N", [1, 4, 9, 13])
+def test_dense_m2m_equals_oracle_m2m(lfmm, orc, N):
+    """The matrix-core M2M operator of child class c (bits x y z) maps the
+    child's normalised multipole (box edge 1/2: M_n / 2^-n) to the parent's
+    (edge 1), as the oracle's rotation-based M2M (reference M2M,
+    src/cpu/multipole3d.cpp:115-150) translates it by the child centre minus
+    the parent centre, (+-1/4, +-1/4, +-1/4)."""
+    rng = np.random.default_rng(100 + N)
+    for c in range(8):
+        K = lfmm.host_m2m_dense(N, c)
+        M = random_multipole(N, rng)
+        off = np.array([(c >> 2) & 1, (c >> 1) & 1, c & 1], dtype=float) / 2.0 - 0.25
+        with orc.exact_geometry():
+            Mp = orc.m2m3d(N, M, off)
+        Mn = np.array([M[tri(n) + m] / 0.5 ** n for n in range(N) for m in range(n + 1)])
+        x = to_real(N, Mn)
+        y = K @ x
+        ref = to_real(N, Mp)
+        scale = np.max(np.abs(K) @ np.abs(x))
+        assert np.max(np.abs(y - ref)) <= 5e-12 * scale, (N, c)
