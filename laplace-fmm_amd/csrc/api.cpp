@@ -627,8 +627,17 @@ int fmm3d_solve(lfmm_ctx* ctx, const lfmm_points3d* S, const lfmm_points3d* Tg, 
             L.src = outer[l + 1];
             L.dst = outer[l];
             if (mixed && l >= mfl) L.absmax = xmax + l;
+            // small uniform levels: the 8 child classes over 8 parts, summed
+            // after (a few tiles alone leave the chip idle)
+            if (!adaptive && !L.absmax && (cnt[l] + rtr_tile(N) - 1) / rtr_tile(N) < 128) {
+                L.dsplit = 8;
+                L.size = (int)cnt[l];
+                L.partial = take<double2>(A, 8 * cnt[l] * NX);
+                if (!L.partial) return LFMM_ERR_NOMEM;
+            }
             LFMM_HIP(ev_open());
             LFMM_HIP(launch_rtr(N, 0, DT->rt, L, st));
+            if (L.dsplit > 1) LFMM_HIP(launch_sum_parts(L.partial, L.dsplit, cnt[l] * NX, outer[l], false, st));
             LFMM_HIP(ev_close(EV_M2M));
         }
     }

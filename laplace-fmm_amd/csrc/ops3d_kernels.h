@@ -387,7 +387,9 @@ __device__ __forceinline__ void rtr_wave(RTR_TABLE_PARAMS, const RtrLevel& L, do
             c = r & 7;
         }
     } else {
-        tile = blk;
+        // M2M: the 8 child classes of a tile split over dsplit parts (adjacent blocks)
+        tile = blk / L.dsplit;
+        part = blk - tile * L.dsplit;
         c = 0;
     }
     // parent of this lane: the next of the level, or of the caller's list
@@ -792,8 +794,8 @@ hipError_t do_rtr(int kind, const RtrTables& T, const RtrLevel& L, hipStream_t s
         return hipGetLastError();
     }
     if (kind == KIND_M2M)
-        hipLaunchKernelGGL((k_rtr<N, KIND_M2M>), dim3(tiles), dim3(256), 0, st, T.rot, T.ephi, T.trans, T.dir_theta,
-                           T.dir_trans, T.dir_vec, T.class_dirs, L);
+        hipLaunchKernelGGL((k_rtr<N, KIND_M2M>), dim3(tiles * L.dsplit), dim3(256), 0, st, T.rot, T.ephi, T.trans,
+                           T.dir_theta, T.dir_trans, T.dir_vec, T.class_dirs, L);
     else if (kind == KIND_L2L)
         hipLaunchKernelGGL((k_rtr<N, KIND_L2L>), dim3(tiles * 8 * L.dsplit), dim3(256), 0, st, T.rot, T.ephi, T.trans,
                            T.dir_theta, T.dir_trans, T.dir_vec, T.class_dirs, L);
