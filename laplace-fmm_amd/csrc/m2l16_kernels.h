@@ -26,6 +26,12 @@
 //   C  col l % 32, row 8 * (v / 4) + 4 * (l / 32) + v % 4, v = 0..15
 // A and B use the same (lane, slot) -> K assignment, so the sum over K does
 // not depend on how the hardware numbers K within an instruction.
+//
+// The same kernel (k_m2l16r<N, CTW, true>) runs the M2M into a level on
+// uniform trees: the targets are the parents, the 8 child classes are the
+// directions (reference M2M, src/cpu/multipole3d.cpp:115-150), and each
+// target's class-c child is read from the children level's class-major
+// packing, whose entry for parent t is that child.
 #pragma once
 #include "ops3d_kernels.h"  // xcd_local_block, lds_barrier
 
