@@ -119,21 +119,35 @@ __global__ void __launch_bounds__(256) k_m2l16_pack_cls(const double2* __restric
     const int kh = (int)(i / np1);
     const int kc = kh >> 1, half = kh & 1;
     const double sc = m16_xscale(*xmax);
-    int j = 0, ce = 0;
+    // the parent's children (at most 8, in key order): their classes from
+    // independent key loads, so the 8 classes' coefficient loads can all be
+    // in flight together
+    int cb = 0, ce = 0;
     if (P < num_parents) {
-        j = child_begin[P];
+        cb = child_begin[P];
         ce = child_begin[P + 1];
     }
+    int kcls[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) kcls[i] = cb + i < ce ? (int)(child_keys[cb + i] & 7u) : -1;
+    int off[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int k = kc * 16 + half * 8 + s;
+        off[s] = k < S::NR ? m16_coef_offset(k) : -1;
+    }
+#pragma unroll
     for (int c = 0; c < 8; ++c) {
-        // children are in key order, so classes come in increasing order
-        const int child = (j < ce && (int)(child_keys[j] & 7u) == c) ? j++ : -1;
+        int child = -1;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (kcls[i] == c) child = cb + i;
         m16_h8 hv = {}, lv = {};
         if (child >= 0) {
             const double* src = (const double*)(outer + (int64_t)child * stride);
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
-                const int k = kc * 16 + half * 8 + s;
-                const double x = k < S::NR ? src[m16_coef_offset(k)] * sc : 0.0;
+                const double x = off[s] >= 0 ? src[off[s]] * sc : 0.0;
                 const _Float16 h = (_Float16)x;
                 hv[s] = h;
                 lv[s] = (_Float16)(x - (double)h);
