@@ -604,7 +604,11 @@ __global__ void __launch_bounds__(256) k_rtr(RTR_TABLE_PARAMS, RtrLevel L)
 // The field is the Cartesian gradient, so it stays finite on the box axis
 // where the reference's spherical form divides by sin(theta)
 // (src/cpu/multipole3d.cpp:1205-1206).
-template <int N>
+// A compiler memory fence every CF columns of m bounds how far coefficient
+// loads are merged and hoisted: CF = 1 keeps each column's loads at their use
+// (fewest registers, but the m +- 1 neighbours are loaded again per column);
+// larger CF loads more coefficients once, with more registers live.
+template <int N, int CF = 1>
 __device__ __forceinline__ void l2p_eval(const double2* __restrict__ Lt, const double* __restrict__ knm, double ux, double uy,
                                          double uz, double& phi, double& gx, double& gy, double& gz)
 {
@@ -652,9 +656,9 @@ __device__ __forceinline__ void l2p_eval(const double2* __restrict__ Lt, const d
     double2 smm = make_double2(1.0, 0.0);
     sfor<0, N>([&](auto M) {
         constexpr int m = decltype(M)::value;
-        // compiler memory fence: keeps each column's coefficient loads next to
-        // their use (otherwise they are merged and hoisted into one live set)
-        asm volatile("" ::: "memory");
+        // compiler memory fence (see CF above; without any, every load is
+        // merged and hoisted into one live set)
+        if constexpr (m % CF == 0) asm volatile("" ::: "memory");
         if constexpr (m > 0) {
             const double c = -(double)(2 * m - 1);
             smm = make_double2(c * (ux * smm.x - uy * smm.y), c * (ux * smm.y + uy * smm.x));
@@ -711,7 +715,12 @@ __global__ void __launch_bounds__(256) k_l2p(LeafArgs a, const double2* __restri
         const double ih = ldexp(1.0, level) / scale;
         const int64_t box = ADAPTIVE ? a.leaf_gid[leaf] : leaf;
         double lp, gx, gy, gz;
-        l2p_eval<N>(local + box * tri(N), knm, (t.x - cx) * ih, (t.y - cy) * ih, (t.z - cz) * ih, lp, gx, gy, gz);
+        // uniform tree up to 13 terms: fence every 4 columns (the repeated
+        // coefficient loads, which bound this kernel, drop by half, at 166
+        // VGPRs for 13 terms); above 13 terms that passes 256 VGPRs, and the
+        // adaptive kernel also runs the W-list evaluations: CF = 1 there
+        l2p_eval<N, (ADAPTIVE || N > 13) ? 1 : 4>(local + box * tri(N), knm, (t.x - cx) * ih, (t.y - cy) * ih, (t.z - cz) * ih, lp,
+                                      gx, gy, gz);
         const double ih2 = ih * ih;
         fp = lp * ih;
         fx = -gx * ih2;

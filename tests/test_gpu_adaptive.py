@@ -3,7 +3,11 @@
 The reference has uniform-depth trees only; the adaptive solve adds P2L
 (X lists) and M2P (W lists) to M2L and P2P.  Pinned by:
 - uniform input, where every leaf of the adaptive tree sits on one level:
-  the same tree and the same results, bit for bit, as the uniform solve;
+  the same tree, lists without X or W pairs, and the same results as the
+  uniform solve to round-off (<= 1e-14 relative: the same operators and
+  summation order, but the uniform L2P kernel is compiled with its
+  coefficient loads grouped differently, so the compiler's FMA contraction
+  of the field may differ in the last bits);
 - clustered input against the FP64 direct sum: geometric convergence with
   the number of terms (1.3e-3, 2.4e-5, 8.2e-7, 2.5e-8 at 5, 9, 13, 18 terms);
 - the Plummer sphere of the headline workload, at 300,000 and at the full
@@ -46,7 +50,9 @@ def test_adaptive_uniform_input_equals_uniform_tree(lfmm, request):
           flush=True)
     assert ra["tree_depth"] == ru["tree_depth"] and ra["num_leaves"] == ru["num_leaves"]
     assert ra["xw_pairs"] == 0
-    assert np.array_equal(pa, pu) and np.array_equal(fa, fu)
+    dp, df = rel_l2(pa, pu), rel_l2(fa, fu)
+    print("adaptive vs uniform: potential %.2e field %.2e" % (dp, df), flush=True)
+    assert dp <= 1e-14 and df <= 1e-14
 
 
 def test_adaptive_clustered_converges_to_direct(lfmm, orc, request):
