@@ -104,6 +104,10 @@ __global__ void __launch_bounds__(256) k_m2l16_pack(const double2* __restrict__ 
 // the child is absent and in slot num_parents.  For one direction, the
 // sources of a tile's targets are one class and a shifted block of parents,
 // so a source fragment reads runs of consecutive 16-byte entries.
+// A thread packs one (parent, kc, half): the 2 KC threads of a parent are
+// consecutive, so together they read each child's expansion in one
+// contiguous sweep, and blocks are numbered XCD-locally, so the partial
+// 16-byte rows of neighbouring parents merge in one L2.
 template <int N>
 __global__ void __launch_bounds__(256) k_m2l16_pack_cls(const double2* __restrict__ outer, int num_parents,
                                                         const int32_t* __restrict__ child_begin,
@@ -113,10 +117,10 @@ __global__ void __launch_bounds__(256) k_m2l16_pack_cls(const double2* __restric
 {
     using S = M16<N>;
     const int64_t np1 = (int64_t)num_parents + 1;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (kc, half, parent), parent fastest
+    const int64_t i = (int64_t)xcd_local_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;  // (parent, kc, half)
     if (i >= np1 * S::KC * 2) return;
-    const int64_t P = i % np1;
-    const int kh = (int)(i / np1);
+    const int64_t P = i / (S::KC * 2);
+    const int kh = (int)(i - P * (S::KC * 2));
     const int kc = kh >> 1, half = kh & 1;
     const double sc = m16_xscale(*xmax);
     // the parent's children (at most 8, in key order): their classes from
